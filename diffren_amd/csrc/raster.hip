@@ -1,0 +1,814 @@
+// Layered z-buffer rasterizer for gfx950: screen-tile binning, per-tile
+// coverage and L-deep depth test in LDS, and an in-tile resolve that writes
+// triangle ids, z and perspective-correct barycentrics.
+//
+// Results are bit-identical to diffren's CPU rasterizer
+// (diffren/common/kernels/rasterize_triangles_impl.cc:27-118 with the helpers
+// of rasterize_utils.h:51-248):
+//   * coverage uses the reference's exact int64 16.16 fixed-point edge
+//     functions, evaluated with two's-complement wrap;
+//   * FP32 math is evaluated without contraction (pragma below, and the file is
+//     built with -ffp-contract=off) with correctly rounded division;
+//   * float->int conversions reproduce x86-64 cvttss2si (the platform the
+//     reference CPU kernel is defined on): NaN / out of range -> INT_MIN;
+//   * each pixel keeps the L smallest (z, triangle id) pairs, z compared as
+//     floats (so -0 == +0) with ties going to the lower id, which is what the
+//     reference's id-ordered insertion sort with strict '<' produces
+//     (rasterize_triangles_impl.cc:95-114).  Fragments with z outside
+//     [-1, 1] (impl.cc:92), z == 1 (never below the clear depth) or NaN z are
+//     never stored.
+//
+// Pipeline per call (all jobs of a batch in each launch):
+//   clear -> bin count -> scan (2 kernels) -> bin fill -> tile raster.
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cstring>
+
+#include "dr_common.h"
+#include "../../include/diffren_amd.h"
+
+namespace dr {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxLayers = 8;
+constexpr uint64_t kEmpty = ~uint64_t(0);
+// Clipped-bbox area above which a triangle is rasterized by a whole wave
+// instead of a single lane.
+constexpr int kWaveRasterArea = 24;
+
+// ---------------------------------------------------------------------------
+// Exact reference arithmetic.
+
+__device__ __forceinline__ int f2i_x86(float f) {
+  return (f >= -2147483648.0f && f < 2147483648.0f) ? static_cast<int>(f)
+                                                     : INT32_MIN;
+}
+
+__device__ __forceinline__ int64_t f2l_x86(float f) {
+  return (f >= -9223372036854775808.0f && f < 9223372036854775808.0f)
+             ? static_cast<int64_t>(f)
+             : INT64_MIN;
+}
+
+__device__ __forceinline__ int64_t to_fixed(float f) {
+  return f2l_x86(f * 65536.0f);
+}
+
+__device__ __forceinline__ int64_t wmul(int64_t a, int64_t b) {
+  return static_cast<int64_t>(static_cast<uint64_t>(a) *
+                              static_cast<uint64_t>(b));
+}
+__device__ __forceinline__ int64_t wadd(int64_t a, int64_t b) {
+  return static_cast<int64_t>(static_cast<uint64_t>(a) +
+                              static_cast<uint64_t>(b));
+}
+__device__ __forceinline__ int64_t wsub(int64_t a, int64_t b) {
+  return static_cast<int64_t>(static_cast<uint64_t>(a) -
+                              static_cast<uint64_t>(b));
+}
+
+struct Box {
+  int left, right, bottom, top;
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// rasterize_utils.h:75-84
+__device__ __forceinline__ void grow(float x, float y, float w, int W, int H,
+                                     Box& b) {
+  const float sx = 0.5f * (x / w + 1.0f) * static_cast<float>(W);
+  const float sy = 0.5f * (y / w + 1.0f) * static_cast<float>(H);
+  b.left = clampi(f2i_x86(floorf(fminf(static_cast<float>(b.left), sx))), 0, W);
+  b.right = clampi(f2i_x86(ceilf(fmaxf(static_cast<float>(b.right), sx))), 0, W);
+  b.bottom =
+      clampi(f2i_x86(floorf(fminf(static_cast<float>(b.bottom), sy))), 0, H);
+  b.top = clampi(f2i_x86(ceilf(fmaxf(static_cast<float>(b.top), sy))), 0, H);
+}
+
+// rasterize_utils.h:86-99
+__device__ __forceinline__ void grow_near(const float4& p, const float4& q,
+                                          int W, int H, Box& b) {
+  const float den = p.z - q.z + p.w - q.w;
+  if (den != 0.0f) {
+    const float t = (p.z + p.w) / den;
+    grow(p.x + t * (q.x - p.x), p.y + t * (q.y - p.y), p.w + t * (q.w - p.w),
+         W, H, b);
+  }
+}
+
+__device__ __forceinline__ bool in_front(const float4& v) {
+  return v.w > 0.0f && v.z >= -v.w;
+}
+
+// rasterize_utils.h:111-161
+__device__ __forceinline__ bool triangle_bbox(const float4 v[3], int W, int H,
+                                              Box& b) {
+  b.left = W;
+  b.right = 0;
+  b.bottom = H;
+  b.top = 0;
+  const bool vis0 = in_front(v[0]), vis1 = in_front(v[1]),
+             vis2 = in_front(v[2]);
+  if (vis0) {
+    grow(v[0].x, v[0].y, v[0].w, W, H, b);
+    if (!vis1) grow_near(v[0], v[1], W, H, b);
+    if (!vis2) grow_near(v[0], v[2], W, H, b);
+  }
+  if (vis1) {
+    grow(v[1].x, v[1].y, v[1].w, W, H, b);
+    if (!vis2) grow_near(v[1], v[2], W, H, b);
+    if (!vis0) grow_near(v[1], v[0], W, H, b);
+  }
+  if (vis2) {
+    grow(v[2].x, v[2].y, v[2].w, W, H, b);
+    if (!vis0) grow_near(v[2], v[0], W, H, b);
+    if (!vis1) grow_near(v[2], v[1], W, H, b);
+  }
+  return b.right > b.left && b.top > b.bottom;
+}
+
+// Fixed-point matrix [[x0 x1 x2] [y0 y1 y2] [w0 w1 w2]] (row-major).
+__device__ __forceinline__ void fixed_matrix(const float4 v[3], int64_t m[9]) {
+  m[0] = to_fixed(v[0].x); m[1] = to_fixed(v[1].x); m[2] = to_fixed(v[2].x);
+  m[3] = to_fixed(v[0].y); m[4] = to_fixed(v[1].y); m[5] = to_fixed(v[2].y);
+  m[6] = to_fixed(v[0].w); m[7] = to_fixed(v[1].w); m[8] = to_fixed(v[2].w);
+}
+
+// rasterize_utils.h:176-206.  For kNone/kFront the reference multiplies each
+// int64 entry by a float, i.e. converts the entry to float, multiplies and
+// truncates back; that rounding is reproduced.
+__device__ __forceinline__ void adjugate(const int64_t m[9], int cull,
+                                         int64_t r[9]) {
+  r[0] = wsub(wmul(m[4], m[8]), wmul(m[7], m[5]));
+  r[1] = wsub(wmul(m[2], m[7]), wmul(m[8], m[1]));
+  r[2] = wsub(wmul(m[1], m[5]), wmul(m[4], m[2]));
+  r[3] = wsub(wmul(m[5], m[6]), wmul(m[8], m[3]));
+  r[4] = wsub(wmul(m[0], m[8]), wmul(m[6], m[2]));
+  r[5] = wsub(wmul(m[2], m[3]), wmul(m[5], m[0]));
+  r[6] = wsub(wmul(m[3], m[7]), wmul(m[6], m[4]));
+  r[7] = wsub(wmul(m[1], m[6]), wmul(m[7], m[0]));
+  r[8] = wsub(wmul(m[0], m[4]), wmul(m[3], m[1]));
+  if (cull == DR_CULL_NONE || cull == DR_CULL_FRONT) {
+    const int64_t det_i =
+        wadd(wadd(wmul(m[0], r[0]), wmul(m[1], r[3])), wmul(m[2], r[6]));
+    const float det = static_cast<float>(det_i);
+    const float s = (cull == DR_CULL_NONE) ? copysignf(1.0f, det) : -1.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) r[i] = f2l_x86(static_cast<float>(r[i]) * s);
+  }
+}
+
+// Conservative early rejection of triangles that cannot cover any pixel
+// because of culling.  Exact: it only fires when every fixed-point entry is
+// below 2^20 in magnitude (so det and all edge values are computed without
+// int64 wrap) and all three fixed-point w are positive.  Then, with
+// p = (px, py, 2^16), sum_i w_i * e_i(p) = det * 2^16 for the unrounded
+// adjugate, so det <= 0 leaves no pixel with all e_i >= 0 and one > 0
+// (kBack).  For kFront the adjugate entries were rounded through float;
+// the rejection additionally requires det to exceed the largest possible
+// effect of that rounding on sum_i w_i * e_i(p) over the image.
+__device__ __forceinline__ bool cannot_cover(const int64_t m[9], int cull) {
+  if (cull == DR_CULL_NONE) return false;
+  const int64_t lim = int64_t(1) << 20;
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    if (m[i] >= lim || m[i] <= -lim) return false;
+  if (m[6] <= 0 || m[7] <= 0 || m[8] <= 0) return false;
+  const int64_t r0 = m[4] * m[8] - m[7] * m[5];
+  const int64_t r3 = m[5] * m[6] - m[8] * m[3];
+  const int64_t r6 = m[3] * m[7] - m[6] * m[4];
+  const int64_t det = m[0] * r0 + m[1] * r3 + m[2] * r6;
+  if (cull == DR_CULL_BACK) return det <= 0;
+  // kFront: culled when det > 0 by more than the float rounding can move.
+  if (det <= 0) return false;
+  int64_t r[9];
+  r[0] = r0; r[3] = r3; r[6] = r6;
+  r[1] = m[2] * m[7] - m[8] * m[1];
+  r[2] = m[1] * m[5] - m[4] * m[2];
+  r[4] = m[0] * m[8] - m[6] * m[2];
+  r[5] = m[2] * m[3] - m[5] * m[0];
+  r[7] = m[1] * m[6] - m[7] * m[0];
+  r[8] = m[0] * m[4] - m[3] * m[1];
+  double bound = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double row = fabs(double(r[3 * i])) + fabs(double(r[3 * i + 1])) +
+                       fabs(double(r[3 * i + 2]));
+    bound += double(m[6 + i]) * (row * 0x1p-23 + 3.0);
+  }
+  return double(det) > 2.0 * bound + 1.0;
+}
+
+struct Fragment {
+  float b0, b1, b2, z;
+};
+
+// Edge test at pixel (ix, iy); rasterize_utils.h:220-248 and
+// rasterize_triangles_impl.cc:73-92.  Returns true when the pixel is inside
+// and its z is storable.
+__device__ __forceinline__ bool shade_pixel(int ix, int iy, float sx2,
+                                            float sy2, const int64_t r[9],
+                                            const float4 v[3], Fragment& f) {
+  const float px = sx2 * (static_cast<float>(ix) + 0.5f) - 1.0f;
+  const float py = sy2 * (static_cast<float>(iy) + 0.5f) - 1.0f;
+  const int64_t pxi = to_fixed(px), pyi = to_fixed(py);
+  int64_t e[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    e[i] = wadd(wadd(wmul(r[3 * i], pxi), wmul(r[3 * i + 1], pyi)),
+                static_cast<int64_t>(static_cast<uint64_t>(r[3 * i + 2]) << 16));
+  if (!((e[0] >= 0 && e[1] >= 0 && e[2] >= 0) &&
+        (e[0] > 0 || e[1] > 0 || e[2] > 0)))
+    return false;
+  const float sum = static_cast<float>(wadd(wadd(e[0], e[1]), e[2]));
+  f.b0 = static_cast<float>(e[0]) / sum;
+  f.b1 = static_cast<float>(e[1]) / sum;
+  f.b2 = static_cast<float>(e[2]) / sum;
+  const float cz = f.b0 * v[0].z + f.b1 * v[1].z + f.b2 * v[2].z;
+  const float cw = f.b0 * v[0].w + f.b1 * v[1].w + f.b2 * v[2].w;
+  f.z = cz / cw;
+  // Storable iff -1 <= z < 1 (z == 1 never beats the clear depth; NaN never
+  // compares below anything).
+  return f.z >= -1.0f && f.z < 1.0f;
+}
+
+// Order-preserving 32-bit image of z in [-1, 1) with -0 folded onto +0.
+__device__ __forceinline__ uint32_t z_order(float z) {
+  const uint32_t b = __float_as_uint(z == 0.0f ? 0.0f : z);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+// ---------------------------------------------------------------------------
+// Jobs.
+
+struct Jobs {
+  const dr_raster_job* table;  // device table, or null for a single job
+  dr_raster_job one;
+  int32_t count;
+};
+
+__device__ __forceinline__ dr_raster_job load_job(const Jobs& J, int j) {
+  return J.table ? J.table[j] : J.one;
+}
+
+// Largest j with begin(j) <= flat.
+template <bool kTiles>
+__device__ __forceinline__ int find_job(const Jobs& J, int64_t flat) {
+  if (!J.table) return 0;
+  int lo = 0, hi = J.count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int64_t b =
+        kTiles ? J.table[mid].tile_begin : J.table[mid].triangle_begin;
+    if (b <= flat) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Loads triangle `t` of `job`; false if an index is out of range.
+__device__ __forceinline__ bool load_triangle(const dr_raster_job& job,
+                                              int t, float4 v[3]) {
+  const int32_t* tri = job.triangles + 3 * int64_t(t);
+  const float4* verts = reinterpret_cast<const float4*>(job.vertices);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int32_t i = tri[k];
+    if (i < 0 || i >= job.num_vertices) return false;
+    v[k] = verts[i];
+  }
+  return true;
+}
+
+__device__ __forceinline__ int tiles_x(const dr_raster_job& job, int tw) {
+  return (job.width + tw - 1) / tw;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels.
+
+__global__ void clear_kernel(uint32_t* p, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+
+// Pass 1 (kFill == false): count entries per tile into counter[].
+// Pass 2 (kFill == true): append triangle indices to their tiles' bins, using
+// counter[] as per-tile cursors and offsets[] from the scan.
+// Lanes of a wave that hit the same tile share one atomic (ballot, then ranks
+// from popcounts), which matters because neighbouring triangles of a mesh
+// mostly land in the same tile.
+template <int TW, int TH, bool kFill>
+__global__ __launch_bounds__(kBlock) void bin_kernel(
+    Jobs J, int64_t total_tris, int cull, uint32_t* __restrict__ counter,
+    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ bins,
+    uint64_t capacity) {
+  const int64_t flat = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  int n = 0, tx0 = 0, ty0 = 0, nx = 1, t = 0, ntx = 1;
+  int64_t tile_base = 0;
+  if (flat < total_tris) {
+    const int j = find_job<false>(J, flat);
+    const dr_raster_job job = load_job(J, j);
+    t = int(flat - job.triangle_begin);
+    float4 v[3];
+    Box b;
+    if (load_triangle(job, t, v) &&
+        triangle_bbox(v, job.width, job.height, b)) {
+      int64_t m[9];
+      fixed_matrix(v, m);
+      if (!cannot_cover(m, cull)) {
+        tx0 = b.left / TW;
+        ty0 = b.bottom / TH;
+        nx = (b.right - 1) / TW - tx0 + 1;
+        n = nx * ((b.top - 1) / TH - ty0 + 1);
+        tile_base = job.tile_begin;
+        ntx = tiles_x(job, TW);
+      }
+    }
+  }
+  // Wave-uniform loop over the k-th tile of every lane.
+  for (int k = 0; __ballot(k < n); ++k) {
+    const bool has = k < n;
+    const uint32_t tile =
+        has ? uint32_t(tile_base + int64_t(ty0 + k / nx) * ntx + (tx0 + k % nx))
+            : 0xffffffffu;
+    uint64_t todo = __ballot(has);
+    while (todo) {
+      const int leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+      const uint32_t lt = __builtin_amdgcn_readlane(tile, leader);
+      const bool mine = has && tile == lt;
+      const uint64_t peers = __ballot(mine);
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(&counter[lt], __popcll(peers));
+      base = __builtin_amdgcn_readlane(base, leader);
+      if (kFill && mine) {
+        const uint64_t pos = uint64_t(offsets[lt]) + base +
+                             __popcll(peers & lanemask_lt());
+        if (pos < capacity) bins[pos] = uint32_t(t);
+      }
+      todo &= ~peers;
+    }
+  }
+}
+
+// Exclusive scan of counts[0..n) -> offsets, in two kernels: per-block scans
+// with block totals, then each block adds the sum of the totals before it.
+constexpr int kScanItems = 4;  // per thread
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x,
+                                                         uint32_t* total) {
+  __shared__ uint32_t wave_sums[kBlock / 64];
+  const int lane = lane_id(), wave = threadIdx.x / 64;
+  uint32_t incl = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) wave_sums[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / 64; ++w) {
+    const uint32_t s = wave_sums[w];
+    if (w < wave) before += s;
+    all += s;
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - x;
+}
+
+__global__ __launch_bounds__(kBlock) void scan_local_kernel(
+    const uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets,
+    uint32_t* __restrict__ block_sums, int64_t n) {
+  const int64_t base = int64_t(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+  uint32_t v[kScanItems], s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    v[i] = (base + i < n) ? counts[base + i] : 0u;
+    s += v[i];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(s, &total);
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n) offsets[base + i] = run;
+    run += v[i];
+  }
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
+}
+
+// status[0] = total entries (low 32 bits), status[1] = 1 if > capacity,
+// status[2] = high 32 bits of total.
+__global__ __launch_bounds__(kBlock) void scan_add_kernel(
+    uint32_t* __restrict__ offsets, const uint32_t* __restrict__ block_sums,
+    int nblocks, int64_t n, uint32_t* __restrict__ status, uint64_t capacity) {
+  __shared__ unsigned long long red[kBlock];
+  unsigned long long before = 0, all = 0;
+  for (int i = threadIdx.x; i < nblocks; i += kBlock) {
+    const unsigned long long s = block_sums[i];
+    if (i < int(blockIdx.x)) before += s;
+    all += s;
+  }
+  red[threadIdx.x] = before;
+  __syncthreads();
+  for (int d = kBlock / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
+    __syncthreads();
+  }
+  before = red[0];
+  __syncthreads();
+  red[threadIdx.x] = all;
+  __syncthreads();
+  for (int d = kBlock / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d) red[threadIdx.x] += red[threadIdx.x + d];
+    __syncthreads();
+  }
+  all = red[0];
+  const int64_t base = int64_t(blockIdx.x) * kScanTile;
+  for (int i = threadIdx.x; i < kScanTile; i += kBlock)
+    if (base + i < n) offsets[base + i] += uint32_t(before);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    status[0] = uint32_t(all);
+    status[2] = uint32_t(all >> 32);
+    status[1] = all > capacity ? 1u : 0u;
+  }
+}
+
+// Inserts key into the L-deep sorted list of LDS slots at `p` (slot l at
+// keys[l * TP + p]).  Each slot keeps the minimum it has seen and passes the
+// larger key down, so after all insertions slot l holds the (l+1)-th smallest
+// key regardless of arrival order.
+__device__ __forceinline__ void insert_key(uint64_t* keys, int tp, int p,
+                                          int L, uint64_t key) {
+  for (int l = 0; l < L; ++l) {
+    const uint64_t old = atomicMin(
+        reinterpret_cast<unsigned long long*>(&keys[l * tp + p]),
+        static_cast<unsigned long long>(key));
+    if (old == kEmpty) return;
+    key = old > key ? old : key;
+  }
+}
+
+struct TriRect {
+  int tri;
+  short x0, x1, y0, y1;  // clipped to the tile, image coordinates
+};
+
+// Rasterizes all bin entries of one tile per block, then resolves the tile.
+template <int TW, int TH>
+__global__ __launch_bounds__(kBlock) void raster_tile_kernel(
+    Jobs J, int L, int cull, const uint32_t* __restrict__ counts,
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ bins,
+    uint64_t capacity) {
+  constexpr int TP = TW * TH;
+  extern __shared__ uint64_t keys[];  // [L][TP]
+  __shared__ TriRect queue[kBlock];
+  __shared__ int queue_n;
+
+  const int64_t flat_tile = blockIdx.x;
+  const int j = find_job<true>(J, flat_tile);
+  const dr_raster_job job = load_job(J, j);
+  const int W = job.width, H = job.height;
+  const int ntx = tiles_x(job, TW);
+  const int local = int(flat_tile - job.tile_begin);
+  const int x0 = (local % ntx) * TW, y0 = (local / ntx) * TH;
+  const int x1 = min(x0 + TW, W), y1 = min(y0 + TH, H);
+  const float sx2 = 2.0f / static_cast<float>(W);
+  const float sy2 = 2.0f / static_cast<float>(H);
+
+  for (int i = threadIdx.x; i < L * TP; i += kBlock) keys[i] = kEmpty;
+  const uint32_t cnt = counts[flat_tile];
+  const uint64_t off = offsets[flat_tile];
+  const bool overflow = off + cnt > capacity;
+  const int64_t items = overflow ? int64_t(job.num_triangles) : int64_t(cnt);
+
+  for (int64_t base = 0; base < items; base += kBlock) {
+    if (threadIdx.x == 0) queue_n = 0;
+    __syncthreads();
+    const int64_t i = base + threadIdx.x;
+    bool wave_job = false;
+    TriRect rect;
+    if (i < items) {
+      const int t = overflow ? int(i) : int(bins[off + i]);
+      float4 v[3];
+      Box b;
+      if (load_triangle(job, t, v) && triangle_bbox(v, W, H, b)) {
+        rect.tri = t;
+        rect.x0 = short(max(b.left, x0));
+        rect.x1 = short(min(b.right, x1));
+        rect.y0 = short(max(b.bottom, y0));
+        rect.y1 = short(min(b.top, y1));
+        const int area = max(0, rect.x1 - rect.x0) * max(0, rect.y1 - rect.y0);
+        if (area > kWaveRasterArea) {
+          wave_job = true;
+        } else if (area > 0) {
+          int64_t m[9], r[9];
+          fixed_matrix(v, m);
+          adjugate(m, cull, r);
+          for (int iy = rect.y0; iy < rect.y1; ++iy)
+            for (int ix = rect.x0; ix < rect.x1; ++ix) {
+              Fragment f;
+              if (shade_pixel(ix, iy, sx2, sy2, r, v, f))
+                insert_key(keys, TP, (iy - y0) * TW + (ix - x0), L,
+                           (uint64_t(z_order(f.z)) << 32) | uint32_t(t));
+            }
+        }
+      }
+    }
+    // Queue large triangles: one LDS atomic per wave, ranks from the ballot.
+    const uint64_t mask = __ballot(wave_job);
+    if (mask) {
+      const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+      int qbase = 0;
+      if (lane_id() == leader) qbase = atomicAdd(&queue_n, __popcll(mask));
+      qbase = __builtin_amdgcn_readlane(qbase, leader);
+      if (wave_job) queue[qbase + __popcll(mask & lanemask_lt())] = rect;
+    }
+    __syncthreads();
+    // Each wave walks whole queued triangles, one pixel per lane.
+    const int nq = queue_n;
+    for (int q = threadIdx.x / 64; q < nq; q += kBlock / 64) {
+      const TriRect rq = queue[q];
+      float4 v[3];
+      load_triangle(job, rq.tri, v);
+      int64_t m[9], r[9];
+      fixed_matrix(v, m);
+      adjugate(m, cull, r);
+      const int rw = rq.x1 - rq.x0;
+      const int area = rw * (rq.y1 - rq.y0);
+      for (int k = lane_id(); k < area; k += 64) {
+        const int ix = rq.x0 + k % rw, iy = rq.y0 + k / rw;
+        Fragment f;
+        if (shade_pixel(ix, iy, sx2, sy2, r, v, f))
+          insert_key(keys, TP, (iy - y0) * TW + (ix - x0), L,
+                     (uint64_t(z_order(f.z)) << 32) | uint32_t(rq.tri));
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+
+  // Resolve: recompute the winning fragments exactly and write the outputs.
+  const int64_t P = int64_t(W) * H;
+  for (int p = threadIdx.x; p < TP; p += kBlock) {
+    const int ix = x0 + p % TW, iy = y0 + p / TW;
+    if (ix >= x1 || iy >= y1) continue;
+    const int64_t pix = int64_t(iy) * W + ix;
+    for (int l = 0; l < L; ++l) {
+      const uint64_t key = keys[l * TP + p];
+      const int64_t o = pix + l * P;
+      int id = -1;
+      Fragment f{0.0f, 0.0f, 0.0f, 1.0f};
+      if (key != kEmpty) {
+        id = int(uint32_t(key));
+        float4 v[3];
+        load_triangle(job, id, v);
+        int64_t m[9], r[9];
+        fixed_matrix(v, m);
+        adjugate(m, cull, r);
+        shade_pixel(ix, iy, sx2, sy2, r, v, f);
+      }
+      job.triangle_ids[o] = id;
+      job.z_buffer[o] = f.z;
+      if (job.barycentrics) {
+        job.barycentrics[3 * o + 0] = f.b0;
+        job.barycentrics[3 * o + 1] = f.b1;
+        job.barycentrics[3 * o + 2] = f.b2;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+struct Workspace {
+  uint32_t* counts;
+  uint32_t* offsets;
+  uint32_t* block_sums;
+  uint32_t* status;
+  uint32_t* bins;
+  size_t bytes;
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+inline bool valid_tile(int tw, int th) {
+  return (tw == 8 && th == 8) || (tw == 16 && th == 16) ||
+         (tw == 32 && th == 8) || (tw == 32 && th == 16) ||
+         (tw == 32 && th == 32) || (tw == 64 && th == 8);
+}
+
+inline int64_t scan_blocks(int64_t total_tiles) {
+  return (total_tiles + kScanTile - 1) / kScanTile;
+}
+
+Workspace carve(void* base, int64_t total_tiles, uint64_t capacity) {
+  Workspace w;
+  char* p = static_cast<char*>(base);
+  size_t o = 0;
+  w.status = reinterpret_cast<uint32_t*>(p + o);
+  o += align256(4 * sizeof(uint32_t));
+  w.counts = reinterpret_cast<uint32_t*>(p + o);
+  o += align256(sizeof(uint32_t) * total_tiles);
+  w.offsets = reinterpret_cast<uint32_t*>(p + o);
+  o += align256(sizeof(uint32_t) * total_tiles);
+  w.block_sums = reinterpret_cast<uint32_t*>(p + o);
+  o += align256(sizeof(uint32_t) * scan_blocks(total_tiles));
+  w.bins = reinterpret_cast<uint32_t*>(p + o);
+  o += align256(sizeof(uint32_t) * capacity);
+  w.bytes = o;
+  return w;
+}
+
+uint64_t capacity_of(const dr_raster_config& c, int64_t total_tris,
+                     int64_t total_tiles) {
+  if (c.bin_capacity > 0) return uint64_t(c.bin_capacity);
+  return uint64_t(3 * total_tris + 16 * total_tiles + 4096);
+}
+
+template <int TW, int TH>
+int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
+               int cull, const Workspace& w, uint64_t capacity,
+               hipStream_t s) {
+  const int64_t clear_n = total_tiles;
+  clear_kernel<<<dim3(unsigned((clear_n + kBlock - 1) / kBlock)), kBlock, 0,
+                 s>>>(w.counts, clear_n);
+  DR_CHECK_LAUNCH("raster clear");
+  const unsigned tri_blocks = unsigned((total_tris + kBlock - 1) / kBlock);
+  if (tri_blocks > 0) {
+    bin_kernel<TW, TH, false><<<tri_blocks, kBlock, 0, s>>>(
+        J, total_tris, cull, w.counts, w.offsets, w.bins, capacity);
+    DR_CHECK_LAUNCH("raster bin count");
+  }
+  const int64_t sb = scan_blocks(total_tiles);
+  scan_local_kernel<<<unsigned(sb), kBlock, 0, s>>>(w.counts, w.offsets,
+                                                   w.block_sums, total_tiles);
+  DR_CHECK_LAUNCH("raster scan");
+  scan_add_kernel<<<unsigned(sb), kBlock, 0, s>>>(
+      w.offsets, w.block_sums, int(sb), total_tiles, w.status, capacity);
+  DR_CHECK_LAUNCH("raster scan add");
+  // Second pass reuses counts as cursors after copying them away: the tile
+  // kernel needs the final counts, which equal the cursors after the fill.
+  clear_kernel<<<dim3(unsigned((clear_n + kBlock - 1) / kBlock)), kBlock, 0,
+                 s>>>(w.counts, clear_n);
+  DR_CHECK_LAUNCH("raster clear cursors");
+  if (tri_blocks > 0) {
+    bin_kernel<TW, TH, true><<<tri_blocks, kBlock, 0, s>>>(
+        J, total_tris, cull, w.counts, w.offsets, w.bins, capacity);
+    DR_CHECK_LAUNCH("raster bin fill");
+  }
+  const size_t lds = sizeof(uint64_t) * L * TW * TH;
+  raster_tile_kernel<TW, TH><<<unsigned(total_tiles), kBlock, lds, s>>>(
+      J, L, cull, w.counts, w.offsets, w.bins, capacity);
+  DR_CHECK_LAUNCH("raster tiles");
+  return kOk;
+}
+
+int dispatch(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
+             int cull, const dr_raster_config& c, void* ws, size_t ws_bytes,
+             hipStream_t s) {
+  DR_CHECK_ARG(L >= 1 && L <= kMaxLayers, "num_layers must be in [1, 8]");
+  DR_CHECK_ARG(cull >= 0 && cull <= 2, "face_culling_mode must be 0, 1 or 2");
+  DR_CHECK_ARG(valid_tile(c.tile_w, c.tile_h), "unsupported tile size");
+  DR_CHECK_ARG(total_tiles > 0 && total_tiles < (int64_t(1) << 31),
+               "tile count out of range");
+  DR_CHECK_ARG(total_tris >= 0 && total_tris < (int64_t(1) << 40),
+               "triangle count out of range");
+  DR_CHECK_ARG(size_t(L) * c.tile_w * c.tile_h * sizeof(uint64_t) <= 65536,
+               "num_layers x tile pixels exceeds the 64 KiB LDS key budget");
+  const uint64_t capacity = capacity_of(c, total_tris, total_tiles);
+  DR_CHECK_ARG(capacity < (uint64_t(1) << 32), "bin capacity must be < 2^32");
+  DR_CHECK_ARG(ws != nullptr, "workspace is null");
+  DR_CHECK_ARG((reinterpret_cast<uintptr_t>(ws) & 255) == 0,
+               "workspace must be 256-byte aligned");
+  const Workspace w = carve(ws, total_tiles, capacity);
+  if (ws_bytes < w.bytes)
+    return fail(kWorkspaceTooSmall,
+                "workspace too small: need " + std::to_string(w.bytes) +
+                    " bytes, got " + std::to_string(ws_bytes));
+#define DR_TILE_CASE(A, B)                                              \
+  if (c.tile_w == A && c.tile_h == B)                                   \
+    return launch_all<A, B>(J, total_tris, total_tiles, L, cull, w,     \
+                            capacity, s);
+  DR_TILE_CASE(8, 8)
+  DR_TILE_CASE(16, 16)
+  DR_TILE_CASE(32, 8)
+  DR_TILE_CASE(32, 16)
+  DR_TILE_CASE(32, 32)
+  DR_TILE_CASE(64, 8)
+#undef DR_TILE_CASE
+  return fail(kUnsupported, "unsupported tile size");
+}
+
+dr_raster_config resolve_config(const dr_raster_config* c) {
+  dr_raster_config r{32, 8, 0};
+  if (c) r = *c;
+  return r;
+}
+
+}  // namespace
+}  // namespace dr
+
+extern "C" {
+
+int dr_raster_jobs_prepare(dr_raster_job* jobs, int32_t n,
+                           const dr_raster_config* config,
+                           int64_t* total_triangles, int64_t* total_tiles) {
+  const dr_raster_config c = dr::resolve_config(config);
+  DR_CHECK_ARG(jobs != nullptr && n > 0, "need at least one job");
+  DR_CHECK_ARG(dr::valid_tile(c.tile_w, c.tile_h), "unsupported tile size");
+  int64_t tri = 0, tiles = 0;
+  for (int i = 0; i < n; ++i) {
+    dr_raster_job& j = jobs[i];
+    DR_CHECK_ARG(j.width > 0 && j.height > 0, "image size must be > 0");
+    DR_CHECK_ARG(j.width <= 32767 && j.height <= 32767,
+                 "image size must be <= 32767");
+    DR_CHECK_ARG(j.num_triangles >= 0 && j.num_vertices >= 0,
+                 "negative mesh size");
+    DR_CHECK_ARG(j.num_triangles == 0 || (j.vertices && j.triangles),
+                 "null mesh pointer");
+    DR_CHECK_ARG(j.triangle_ids && j.z_buffer, "null output pointer");
+    DR_CHECK_ARG((reinterpret_cast<uintptr_t>(j.vertices) & 15) == 0,
+                 "vertices must be 16-byte aligned");
+    j.triangle_begin = tri;
+    j.tile_begin = tiles;
+    tri += j.num_triangles;
+    tiles += int64_t((j.width + c.tile_w - 1) / c.tile_w) *
+             ((j.height + c.tile_h - 1) / c.tile_h);
+  }
+  if (total_triangles) *total_triangles = tri;
+  if (total_tiles) *total_tiles = tiles;
+  return DR_OK;
+}
+
+size_t dr_raster_workspace_size(int64_t total_triangles, int64_t total_tiles,
+                                const dr_raster_config* config) {
+  const dr_raster_config c = dr::resolve_config(config);
+  return dr::carve(nullptr, total_tiles,
+                   dr::capacity_of(c, total_triangles, total_tiles))
+      .bytes;
+}
+
+int dr_rasterize_triangles(const float* vertices, int32_t num_vertices,
+                           const int32_t* triangles, int32_t num_triangles,
+                           int32_t width, int32_t height, int32_t num_layers,
+                           int32_t face_culling_mode, int32_t* triangle_ids,
+                           float* z_buffer, float* barycentrics,
+                           const dr_raster_config* config, void* workspace,
+                           size_t workspace_bytes, dr_stream_t stream) {
+  const dr_raster_config c = dr::resolve_config(config);
+  dr_raster_job job{};
+  job.vertices = vertices;
+  job.triangles = triangles;
+  job.triangle_ids = triangle_ids;
+  job.z_buffer = z_buffer;
+  job.barycentrics = barycentrics;
+  job.num_vertices = num_vertices;
+  job.num_triangles = num_triangles;
+  job.width = width;
+  job.height = height;
+  int64_t tt = 0, tiles = 0;
+  const int rc = dr_raster_jobs_prepare(&job, 1, &c, &tt, &tiles);
+  if (rc != DR_OK) return rc;
+  dr::Jobs J{nullptr, job, 1};
+  return dr::dispatch(J, tt, tiles, num_layers, face_culling_mode, c,
+                      workspace, workspace_bytes,
+                      static_cast<hipStream_t>(stream));
+}
+
+int dr_rasterize_triangles_batched(const dr_raster_job* jobs_device,
+                                   int32_t num_jobs, int64_t total_triangles,
+                                   int64_t total_tiles, int32_t num_layers,
+                                   int32_t face_culling_mode,
+                                   const dr_raster_config* config,
+                                   void* workspace, size_t workspace_bytes,
+                                   dr_stream_t stream) {
+  const dr_raster_config c = dr::resolve_config(config);
+  DR_CHECK_ARG(jobs_device != nullptr && num_jobs > 0, "need jobs");
+  dr::Jobs J{jobs_device, dr_raster_job{}, num_jobs};
+  return dr::dispatch(J, total_triangles, total_tiles, num_layers,
+                      face_culling_mode, c, workspace, workspace_bytes,
+                      static_cast<hipStream_t>(stream));
+}
+
+int dr_raster_bin_stats(const void* workspace, int64_t* entries_needed,
+                        int32_t* overflowed, dr_stream_t stream) {
+  DR_CHECK_ARG(workspace != nullptr, "workspace is null");
+  uint32_t st[4] = {0, 0, 0, 0};
+  DR_CHECK_HIP(hipMemcpyAsync(st, workspace, sizeof(st), hipMemcpyDeviceToHost,
+                              static_cast<hipStream_t>(stream)));
+  DR_CHECK_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+  if (entries_needed) *entries_needed = int64_t(st[0]) | (int64_t(st[2]) << 32);
+  if (overflowed) *overflowed = int32_t(st[1]);
+  return DR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,213 @@
+/*
+ * diffren_amd C ABI: MI355X (gfx950) kernels for diffren's
+ * rasterize -> interpolate -> splat path.
+ *
+ * Conventions for every entry point:
+ *   - All array arguments are DEVICE pointers, except where a comment says
+ *     "host".  Nothing is allocated or freed inside an entry point; scratch
+ *     memory comes in through `workspace` and its size is given by the
+ *     matching *_workspace_size() query.
+ *   - Work is enqueued on `stream` (a hipStream_t; NULL is the default stream)
+ *     and the call returns without synchronising, so calls can be captured
+ *     into a hipGraph.
+ *   - The return value is DR_OK (0) or an error code; dr_last_error() gives
+ *     the message for the calling thread.
+ *   - Layouts are the reference's: layer-major [L][H][W] buffers, channels
+ *     last, row-major C order, float32 / int32.
+ */
+#ifndef DIFFREN_AMD_H_
+#define DIFFREN_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* dr_stream_t; /* hipStream_t */
+
+enum {
+  DR_OK = 0,
+  DR_INVALID_ARGUMENT = 1,
+  DR_WORKSPACE_TOO_SMALL = 2,
+  DR_LAUNCH_FAILED = 3,
+  DR_UNSUPPORTED = 4
+};
+
+/* Face culling, values of diffren's FaceCullingMode
+ * (diffren/common/kernels/rasterize_triangles_types.h:24,
+ *  diffren/jax/constants.py:20-24). */
+enum { DR_CULL_NONE = 0, DR_CULL_BACK = 1, DR_CULL_FRONT = 2 };
+
+/* Message for the last failed call on this thread ("" if none). */
+const char* dr_last_error(void);
+
+/* Library version string. */
+const char* dr_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Rasterizer                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* One rasterization job (one mesh seen through one camera).  `vertices` are
+ * clip-space xyzw.  Outputs are written completely: background pixels get
+ * triangle id -1, z 1.0 and barycentrics 0, as the reference CPU handler
+ * initialises them (rasterize_triangles_cpu.cc:55-60).  `barycentrics` may be
+ * NULL.  triangle_begin / tile_begin are filled by dr_raster_jobs_prepare. */
+typedef struct dr_raster_job {
+  const float* vertices;   /* [num_vertices, 4] */
+  const int32_t* triangles; /* [num_triangles, 3] */
+  int32_t* triangle_ids;   /* [L, height, width] */
+  float* z_buffer;         /* [L, height, width] */
+  float* barycentrics;     /* [L, height, width, 3] or NULL */
+  int32_t num_vertices;
+  int32_t num_triangles;
+  int32_t width;
+  int32_t height;
+  int64_t triangle_begin; /* exclusive prefix of num_triangles over jobs */
+  int64_t tile_begin;     /* exclusive prefix of screen tiles over jobs */
+} dr_raster_job;
+
+/* Screen-tile binning parameters (chosen by the autotuner).
+ * tile_w x tile_h must be one of 8x8, 16x16, 32x8, 32x16, 32x32, 64x8.
+ * bin_capacity is the number of (tile, triangle) entries the workspace holds;
+ * if a frame needs more, the tiles past the capacity fall back to testing
+ * every triangle of their job (correct, slow) and dr_raster_overflowed()
+ * reports it.  0 selects 2 * triangles + 4096. */
+typedef struct dr_raster_config {
+  int32_t tile_w;
+  int32_t tile_h;
+  int64_t bin_capacity;
+} dr_raster_config;
+
+/* Host: fills triangle_begin / tile_begin of jobs_host[0..num_jobs) and
+ * returns the totals.  Returns DR_OK or DR_INVALID_ARGUMENT. */
+int dr_raster_jobs_prepare(dr_raster_job* jobs_host, int32_t num_jobs,
+                           const dr_raster_config* config,
+                           int64_t* total_triangles, int64_t* total_tiles);
+
+/* Workspace bytes for a call with these totals. */
+size_t dr_raster_workspace_size(int64_t total_triangles, int64_t total_tiles,
+                                const dr_raster_config* config);
+
+/* Rasterizes one clip-space mesh into L depth layers.
+ * Replaces the reference's XLA FFI target "rasterize_triangles"
+ * (diffren/jax/internal/kernels/rasterize_triangles_gpu.cc:44-77, CPU twin
+ * rasterize_triangles_cpu.cc:43-79) and its CUDA implementation
+ * diffren::cuda::RasterizeTrianglesImpl (rasterize_triangles_impl_cuda.cu.h:35-42).
+ * Triangle ids, z and barycentrics are bit-identical to the reference CPU
+ * kernel diffren::RasterizeTrianglesImpl (rasterize_triangles_impl.cc:27-118).
+ * Triangles with a vertex index outside [0, num_vertices) are skipped.
+ * num_layers must be in [1, 8]. */
+int dr_rasterize_triangles(const float* vertices, int32_t num_vertices,
+                           const int32_t* triangles, int32_t num_triangles,
+                           int32_t width, int32_t height, int32_t num_layers,
+                           int32_t face_culling_mode, int32_t* triangle_ids,
+                           float* z_buffer, float* barycentrics,
+                           const dr_raster_config* config, void* workspace,
+                           size_t workspace_bytes, dr_stream_t stream);
+
+/* Rasterizes many (mesh, camera, resolution) jobs in one set of launches.
+ * jobs_device: device copy of a table prepared by dr_raster_jobs_prepare.
+ * No reference counterpart: the reference batches with
+ * vmap_method='sequential_unrolled', one launch per image
+ * (rasterize_triangles_xla.py:68-75). */
+int dr_rasterize_triangles_batched(const dr_raster_job* jobs_device,
+                                   int32_t num_jobs, int64_t total_triangles,
+                                   int64_t total_tiles, int32_t num_layers,
+                                   int32_t face_culling_mode,
+                                   const dr_raster_config* config,
+                                   void* workspace, size_t workspace_bytes,
+                                   dr_stream_t stream);
+
+/* Host: reads back (synchronously, on `stream`) the binning statistics of the
+ * last call that used `workspace`: entries needed and whether the capacity
+ * overflowed.  For the autotuner and tests; not for hot loops. */
+int dr_raster_bin_stats(const void* workspace, int64_t* entries_needed,
+                        int32_t* overflowed, dr_stream_t stream);
+
+
+/* ------------------------------------------------------------------------ */
+/* Vertex transform                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* out[v] = M @ [x, y, z, (w or 1)] for a row-major [4,4] device `matrix`,
+ * summed in the fixed order ((m0 x + m1 y) + m2 z) + m3 w.
+ * Replaces transforms.transform_homogeneous (diffren/jax/utils/transforms.py:
+ * 40-78) as used by rasterize.py:90-96 and splat.py:209-212.
+ * vertices [num_vertices, dims] (dims 3 or 4), out [num_vertices, 4]. */
+int dr_transform_homogeneous(const float* matrix, const float* vertices,
+                             int32_t dims, int64_t num_vertices, float* out,
+                             dr_stream_t stream);
+
+/* d_vertices[v, j] = sum_i M[i, j] d_clip[v, i], j < dims (overwrites). */
+int dr_transform_homogeneous_bwd(const float* matrix, const float* d_clip,
+                                 int32_t dims, int64_t num_vertices,
+                                 float* d_vertices, dr_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Interpolation                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* out[p, c] = (b0 A[v0,c] + b1 A[v1,c]) + b2 A[v2,c] where triangle_ids[p]
+ * is a triangle (v0,v1,v2) = triangles[id], else background[c] (0 if
+ * background is NULL).  num_pixels = L*H*W pixel-layers; barycentrics
+ * [num_pixels, 3]; attributes [num_vertices, channels]; out [num_pixels,
+ * channels].  Replaces interpolate._interpolate_vertex_attribute_layer and
+ * interpolate_vertex_attribute (diffren/jax/internal/interpolate.py:24-63,
+ * 118-145), including the vertex_ids gather of rasterize.py:111-113. */
+int dr_interpolate_fwd(const int32_t* triangle_ids, const float* barycentrics,
+                       const int32_t* triangles, int32_t num_triangles,
+                       const float* attributes, int32_t num_vertices,
+                       int32_t channels, const float* background,
+                       int64_t num_pixels, float* out, dr_stream_t stream);
+
+/* d_attributes[v_k, c] += b_k d_out[p, c] over foreground p (accumulates:
+ * the caller clears d_attributes).  Float atomics: the summation order is
+ * not fixed.  No reference counterpart (the reference uses XLA autodiff of
+ * interpolate.py:49-50). */
+int dr_interpolate_bwd(const int32_t* triangle_ids, const float* barycentrics,
+                       const int32_t* triangles, int32_t num_triangles,
+                       const float* d_out, int32_t num_vertices,
+                       int32_t channels, int64_t num_pixels,
+                       float* d_attributes, dr_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
+/* Splatting                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* The splat coefficient 1.05 / sum_{u,v in -1..1} exp(-2 (u^2+v^2))
+ * (splat.py:63-78), as float. */
+float dr_splat_coefficient(void);
+
+/* 3x3 Gaussian splat of every pixel-layer sample at its pixel centre,
+ * accumulated into foreground / surface / background layers, normalised and
+ * composited with `over`.  Replaces splat.splat_at_pixel_centers
+ * (diffren/jax/internal/splat.py:26-175).
+ *   xyz [L,H,W,3] viewport positions, color [L,H,W,C], mask [L,H,W]
+ *   out [H,W,C]; optional (NULL to skip): accum [3,H,W,C], norm [3,H,W,C],
+ *   accum_weights [3,H,W].  C in [1, 16], L in [1, 8]. */
+int dr_splat_fwd(const float* xyz, const float* color, const float* mask,
+                 int32_t num_layers, int32_t height, int32_t width,
+                 int32_t channels, float* out, float* accum, float* norm,
+                 float* accum_weights, dr_stream_t stream);
+
+/* Workspace bytes for dr_splat_bwd: H*W*3*(C+1) floats. */
+size_t dr_splat_bwd_workspace_size(int32_t height, int32_t width,
+                                   int32_t channels);
+
+/* Vector-Jacobian product of dr_splat_fwd's `out`: writes d_color [L,H,W,C]
+ * and d_xyz [L,H,W,3] (z component 0: z only selects layers).  No reference
+ * counterpart (the reference uses XLA autodiff of splat.py:26-175). */
+int dr_splat_bwd(const float* xyz, const float* color, const float* mask,
+                 int32_t num_layers, int32_t height, int32_t width,
+                 int32_t channels, const float* d_out, float* d_color,
+                 float* d_xyz, void* workspace, size_t workspace_bytes,
+                 dr_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIFFREN_AMD_H_ */

@@ -1,0 +1,145 @@
+"""Interpolation and splat kernels (forward and hand-written backward) vs the
+CPU oracle (oracle/splat_oracle.py).
+
+Tolerances: forward images within 1e-5 relative (max-abs error over the
+max-abs value); gradients within 1e-4 relative of the float64 oracle.
+"""
+import numpy as np
+import pytest
+import torch
+
+import helpers
+from diffren_amd import interpolate
+from diffren_amd import rasterize
+from diffren_amd import render
+from diffren_amd import scenes
+from diffren_amd import splat
+from diffren_amd import transforms
+from oracle import raster_oracle
+from oracle import splat_oracle as so
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+
+def scene(seed, w, h, layers, subdivisions=3, cull=1):
+  s = scenes.icosphere_scene(seed=seed, subdivisions=subdivisions)
+  clip = scenes.to_clip(s["vertices"], s["camera"])
+  ids, z, b = raster_oracle.rasterize(clip, s["triangles"], w, h, layers,
+                                      cull)
+  return s, clip, ids, z, b
+
+
+def test_transform_matches_oracle(device):
+  s = scenes.icosphere_scene(seed=0)
+  v = torch.as_tensor(s["vertices"], device=device)
+  m = torch.as_tensor(s["camera"], device=device)
+  got = transforms.transform_homogeneous(m, v).cpu()
+  want = so.transform(s["camera"], s["vertices"], torch.float32)
+  assert torch.equal(got, want)  # same order, no contraction: bit-exact
+
+
+@pytest.mark.parametrize("channels", [1, 3, 4, 11])
+def test_interpolate_forward_and_backward(channels, device):
+  s, clip, ids, z, b = scene(1, 96, 64, 3, cull=0)
+  rng = np.random.default_rng(channels)
+  attr = rng.normal(size=(clip.shape[0], channels)).astype(np.float32)
+  fbuf = rasterize.rasterize_triangles(
+      torch.as_tensor(clip, device=device),
+      torch.as_tensor(s["triangles"], device=device), None, 96, 64, 3, 0)
+  a = torch.as_tensor(attr, device=device, dtype=torch.float32)
+  a.requires_grad_(True)
+  bg = rng.normal(size=channels).astype(np.float32)
+  out = interpolate.interpolate_vertex_attribute(a, fbuf, bg)
+  want = so.interpolate(torch.as_tensor(attr), ids, b, s["triangles"], bg)
+  assert helpers.rel_err(out.detach().cpu(), want) <= FWD_TOL
+  g = rng.normal(size=out.shape).astype(np.float32)
+  out.backward(torch.as_tensor(g, device=device))
+  a64 = torch.as_tensor(attr, dtype=torch.float64).requires_grad_(True)
+  w64 = so.interpolate(a64, ids, b, s["triangles"], bg)
+  w64.backward(torch.as_tensor(g, dtype=torch.float64))
+  assert helpers.rel_err(a.grad.cpu(), a64.grad) <= GRAD_TOL
+
+
+def _splat_inputs(seed, w, h, layers, channels):
+  s, clip, ids, z, b = scene(seed, w, h, layers)
+  cbuf = so.interpolate(torch.as_tensor(clip), ids, b, s["triangles"],
+                        (0, 0, 1, 1))
+  xyz = so.viewport_positions(cbuf, w, h).numpy()
+  rng = np.random.default_rng(seed)
+  color = rng.uniform(0, 1, size=(layers, h, w, channels)).astype(np.float32)
+  mask = (ids >= 0).astype(np.float32)[..., None]
+  return xyz.astype(np.float32), color, mask
+
+
+@pytest.mark.parametrize("layers,channels", [(1, 4), (2, 4), (3, 4), (3, 12),
+                                             (4, 2)])
+def test_splat_forward(layers, channels, device):
+  xyz, color, mask = _splat_inputs(layers + channels, 80, 72, layers,
+                                   channels)
+  out, acc, norm = splat.splat_at_pixel_centers(
+      torch.as_tensor(xyz, device=device),
+      torch.as_tensor(color, device=device),
+      torch.as_tensor(mask, device=device))
+  w_out, w_acc, w_norm = so.splat_at_pixel_centers(
+      torch.as_tensor(xyz), torch.as_tensor(color), torch.as_tensor(mask))
+  assert helpers.rel_err(out.cpu(), w_out) <= FWD_TOL
+  assert helpers.rel_err(acc.cpu(), w_acc) <= FWD_TOL
+  assert helpers.rel_err(norm.cpu(), w_norm) <= FWD_TOL
+
+
+@pytest.mark.parametrize("layers,channels", [(1, 4), (3, 4), (3, 12)])
+def test_splat_backward_vs_float64_autodiff(layers, channels, device):
+  xyz, color, mask = _splat_inputs(7 * layers + channels, 48, 40, layers,
+                                   channels)
+  x = torch.as_tensor(xyz, device=device).requires_grad_(True)
+  c = torch.as_tensor(color, device=device).requires_grad_(True)
+  out, _, _ = splat.splat_at_pixel_centers(x, c,
+                                           torch.as_tensor(mask,
+                                                           device=device))
+  g = np.random.default_rng(0).normal(size=out.shape).astype(np.float32)
+  out.backward(torch.as_tensor(g, device=device))
+  x64 = torch.as_tensor(xyz, dtype=torch.float64).requires_grad_(True)
+  c64 = torch.as_tensor(color, dtype=torch.float64).requires_grad_(True)
+  o64, _, _ = so.splat_at_pixel_centers(x64, c64,
+                                        torch.as_tensor(mask,
+                                                        dtype=torch.float64))
+  o64.backward(torch.as_tensor(g, dtype=torch.float64))
+  assert helpers.rel_err(c.grad.cpu(), c64.grad) <= GRAD_TOL
+  assert helpers.rel_err(x.grad.cpu()[..., :2], x64.grad[..., :2]) <= GRAD_TOL
+
+
+def _render_gpu(s, w, h, layers, device, colors=None):
+  v = torch.as_tensor(s["vertices"], device=device).requires_grad_(True)
+  col = torch.as_tensor(s["colors"] if colors is None else colors,
+                        device=device).requires_grad_(True)
+  cam = torch.as_tensor(s["camera"], device=device)
+  tris = torch.as_tensor(s["triangles"], device=device)
+  img = render.render_triangles(v, {"c": col}, tris, cam, w, h,
+                                lambda a: a["c"], num_layers=layers)
+  return img, v, col
+
+
+@pytest.mark.parametrize("layers", [1, 3])
+def test_render_splat_forward_and_gradients(layers, device):
+  w = h = 128
+  s = scenes.icosphere_scene(seed=4)
+  img, v, col = _render_gpu(s, w, h, layers, device)
+  target = torch.rand_like(img)
+  loss = ((img - target) ** 2).mean()
+  loss.backward()
+  # Oracle: rasterize on the CPU from the same clip-space vertices the GPU
+  # rasterized (the kernel transform is bit-exact to so.transform float32).
+  clip32 = so.transform(s["camera"], s["vertices"], torch.float32).numpy()
+  ids, z, b = raster_oracle.rasterize(clip32, s["triangles"], w, h, layers, 1)
+  v64 = torch.as_tensor(s["vertices"], dtype=torch.float64).requires_grad_(
+      True)
+  c64 = torch.as_tensor(s["colors"], dtype=torch.float64).requires_grad_(True)
+  o64 = so.render_splat(v64, c64, s["camera"], ids, b, s["triangles"])
+  assert helpers.rel_err(img.detach().cpu(), o64.detach()) <= FWD_TOL
+  l64 = ((o64 - target.cpu().double()) ** 2).mean()
+  l64.backward()
+  assert helpers.rel_err(col.grad.cpu(), c64.grad) <= GRAD_TOL
+  assert helpers.rel_err(v.grad.cpu(), v64.grad) <= GRAD_TOL

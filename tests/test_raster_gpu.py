@@ -1,0 +1,152 @@
+"""GPU tile rasterizer vs the oracle: triangle ids, z and barycentrics must be
+bit-identical (rasterize_triangles_impl.cc:27-118 semantics)."""
+import hashlib
+
+import numpy as np
+import pytest
+import torch
+
+import helpers
+import scenes as golden_scenes
+from diffren_amd import rasterize
+from diffren_amd import scenes
+from oracle import raster_oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = np.load(helpers.os.path.join(helpers.os.path.dirname(__file__),
+                                      "golden", "raster_golden.npz"))
+CASES = sorted({k.split("/")[0] for k in GOLDEN.files})
+TILES = [(8, 8), (16, 16), (32, 8), (32, 16), (32, 32), (64, 8)]
+
+
+def _digest(a):
+  return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def gpu_raster(v, t, w, h, l, cull, device, **kw):
+  print(f"raster V={len(v)} T={len(t)} {w}x{h} L={l} cull={cull} {kw}",
+        flush=True)
+  vt = torch.as_tensor(np.ascontiguousarray(v, np.float32), device=device)
+  tt = torch.as_tensor(np.ascontiguousarray(t, np.int32), device=device)
+  ids, z, b = rasterize.rasterize_triangles_primitive(vt, tt, w, h, l, cull,
+                                                      **kw)
+  torch.cuda.synchronize()
+  return ids.cpu().numpy(), z.cpu().numpy(), b.cpu().numpy()
+
+
+def assert_same(got, want):
+  for g, w, name in zip(got, want, ("ids", "z", "barycentrics")):
+    if not helpers.same_bits(g, w):
+      bad = np.argwhere(g.view(np.uint32) != w.view(np.uint32)
+                        if g.dtype != np.int32 else g != w)
+      raise AssertionError(f"{name} differ at {len(bad)} places, first "
+                           f"{bad[:4].tolist()}")
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_golden_vectors_bit_exact(name, device):
+  v = GOLDEN[f"{name}/vertices"]
+  t = GOLDEN[f"{name}/triangles"]
+  w, h, l, cull = (int(x) for x in GOLDEN[f"{name}/params"])
+  ids, z, b = gpu_raster(v, t, w, h, l, cull, device)
+  assert [_digest(ids), _digest(z), _digest(b)] == list(
+      GOLDEN[f"{name}/sha256"])
+
+
+@pytest.mark.parametrize("tile", TILES)
+def test_tile_shapes_bit_exact(tile, device):
+  for seed in range(3):
+    v, t = golden_scenes.adversarial_scene(200 + seed, 60, 200)
+    w, h = [(64, 48), (33, 17), (100, 7)][seed]
+    l, cull = 1 + seed, seed
+    assert_same(gpu_raster(v, t, w, h, l, cull, device, tile=tile),
+                raster_oracle.rasterize(v, t, w, h, l, cull))
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_scenes_bit_exact(seed, device):
+  rng = np.random.default_rng(seed)
+  v, t = golden_scenes.adversarial_scene(500 + seed, 50, 300)
+  w = int(rng.choice([1, 5, 16, 33, 128]))
+  h = int(rng.choice([1, 7, 16, 64]))
+  l = int(rng.integers(1, 5))
+  cull = int(rng.integers(0, 3))
+  assert_same(gpu_raster(v, t, w, h, l, cull, device),
+              raster_oracle.rasterize(v, t, w, h, l, cull))
+
+
+def test_bin_overflow_falls_back_exactly(device):
+  s = scenes.icosphere_scene(seed=3)
+  clip = scenes.to_clip(s["vertices"], s["camera"])
+  want = raster_oracle.rasterize(clip, s["triangles"], 96, 80, 2, 0)
+  for cap in (1, 100, 1000):
+    assert_same(gpu_raster(clip, s["triangles"], 96, 80, 2, 0, device,
+                           bin_capacity=cap), want)
+
+
+def test_icosphere_workload_bit_exact(device):
+  s = scenes.icosphere_scene(seed=0)
+  clip = scenes.to_clip(s["vertices"], s["camera"])
+  assert_same(gpu_raster(clip, s["triangles"], 256, 256, 1, 1, device),
+              raster_oracle.rasterize(clip, s["triangles"], 256, 256, 1, 1))
+
+
+def test_million_triangle_sphere_bit_exact(device):
+  v, t = scenes.uv_sphere_mesh(500, 1000)
+  cam = scenes.orbit_cameras(1, seed=0)[0]
+  clip = scenes.to_clip(v, cam)
+  want = raster_oracle.rasterize(clip, t, 1024, 1024, 3, 1)
+  assert_same(gpu_raster(clip, t, 1024, 1024, 3, 1, device), want)
+  assert (want[0][0] >= 0).mean() > 0.2
+
+
+def test_large_triangles_cross_many_tiles(device):
+  # Few triangles covering the whole frame: the wave-per-triangle path.
+  v = np.array([[-1, -1, 0.2, 1], [1, -1, 0.2, 1], [1, 1, -0.3, 1],
+                [-1, 1, 0.5, 1], [0, 0, 0.9, 1]], np.float32)
+  t = np.array([[0, 1, 2], [0, 2, 3], [0, 1, 4], [1, 2, 4], [2, 3, 4]],
+               np.int32)
+  for cull in (0, 1, 2):
+    assert_same(gpu_raster(v, t, 300, 200, 3, cull, device),
+                raster_oracle.rasterize(v, t, 300, 200, 3, cull))
+
+
+def test_batched_jobs_match_single_jobs(device):
+  jobs = []
+  for i, (w, h) in enumerate([(64, 64), (128, 32), (17, 200), (256, 256)]):
+    s = scenes.icosphere_scene(seed=10 + i, subdivisions=2 + i % 2)
+    jobs.append((scenes.to_clip(s["vertices"], s["camera"]), s["triangles"],
+                 w, h))
+  vs = [torch.as_tensor(j[0], device=device) for j in jobs]
+  ts = [torch.as_tensor(j[1], device=device) for j in jobs]
+  out = rasterize.rasterize_batched(vs, ts, [(j[2], j[3]) for j in jobs], 3,
+                                    1)
+  for (v, t, w, h), (ids, z, b) in zip(jobs, out):
+    assert_same((ids.cpu().numpy(), z.cpu().numpy(), b.cpu().numpy()),
+                raster_oracle.rasterize(v, t, w, h, 3, 1))
+
+
+def test_no_barycentrics_and_empty_mesh(device):
+  s = scenes.icosphere_scene(seed=1)
+  clip = scenes.to_clip(s["vertices"], s["camera"])
+  vt = torch.as_tensor(clip, device=device)
+  tt = torch.as_tensor(s["triangles"], device=device)
+  ids, z, _ = rasterize._rasterize_one(vt, tt, 64, 64, 2, 1,
+                                       rasterize._config(), want_barys=False)
+  want = raster_oracle.rasterize(clip, s["triangles"], 64, 64, 2, 1)
+  assert helpers.same_bits(ids.cpu().numpy(), want[0])
+  assert helpers.same_bits(z.cpu().numpy(), want[1])
+  ids, z, b = gpu_raster(clip, np.zeros((0, 3), np.int32), 16, 8, 2, 0, device)
+  assert (ids == -1).all() and (z == 1).all() and (b == 0).all()
+
+
+def test_invalid_arguments_raise(device):
+  vt = torch.zeros((3, 4), device=device)
+  tt = torch.zeros((1, 3), dtype=torch.int32, device=device)
+  with pytest.raises(ValueError):
+    rasterize.rasterize_triangles_primitive(vt, tt, 0, 4, 1, 0)
+  with pytest.raises(ValueError):
+    rasterize.rasterize_triangles_primitive(vt, tt, 4, 4, 9, 0)
+  with pytest.raises(ValueError):
+    rasterize.rasterize_triangles_primitive(vt, tt, 4, 4, 1, 3)
