@@ -1,0 +1,312 @@
+"""Benchmark: batched fused forward+backward rasterize-then-splat.
+
+Workload (the differentiation configuration): a 1,000,000-triangle UV sphere
+(500 x 1000 + 2 poles, 500,002 vertices, 4-octave Perlin displacement of
+amplitude 0.1), RGBA vertex colours, rendered at 1024x1024 with 3 depth layers
+and back-face culling from 8 seeded camera poses on a radius-3 orbit (60 degree
+FOV).  One step = forward render of all 8 views + photometric L2 loss against
+targets rendered from a copy with vertices perturbed by N(0, 0.01) + backward
+to vertex positions and colours.  Inputs are resident in HBM before timing.
+
+Reported next to it, measured in the same run:
+  * the plain path: one render_triangles call per view (one rasterizer launch
+    per image, unfused interpolate / splat kernels, autograd backward), the
+    pattern of the reference's vmap_method='sequential_unrolled';
+  * per-stage kernel times from HIP events and the roofline of the dominant
+    kernel;
+  * the CPU baseline: the reference CPU rasterizer (compiled from the
+    reference sources into oracle/_ref) plus the float32 CPU restatement of
+    interpolate/splat forward+backward, on one view;
+  * the vertex-gradient relative error of one view against the float64 CPU
+    oracle.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+  p = argparse.ArgumentParser()
+  p.add_argument("--gpus", type=int, default=1)
+  p.add_argument("--steps", type=int, default=20)
+  p.add_argument("--warmup", type=int, default=3)
+  p.add_argument("--views", type=int, default=8)
+  p.add_argument("--resolution", type=int, default=1024)
+  p.add_argument("--layers", type=int, default=3)
+  p.add_argument("--rows", type=int, default=500)
+  p.add_argument("--cols", type=int, default=1000)
+  p.add_argument("--plain-steps", type=int, default=3)
+  p.add_argument("--no-cpu-baseline", action="store_true")
+  p.add_argument("--no-grad-check", action="store_true")
+  p.add_argument("--no-graph", action="store_true")
+  return p.parse_args()
+
+
+def log(*a):
+  print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(args, dev):
+  import numpy as np
+  import torch
+  from diffren_amd import scenes
+  v, t = scenes.uv_sphere_mesh(args.rows, args.cols, seed=0)
+  rng = np.random.default_rng(1)
+  colors = np.concatenate([rng.uniform(0, 1, (v.shape[0], 3)),
+                           np.ones((v.shape[0], 1))], 1).astype(np.float32)
+  cams = scenes.orbit_cameras(args.views, seed=0)
+  v_target = (v + rng.normal(0, 0.01, v.shape)).astype(np.float32)
+  mesh = dict(vertices=torch.as_tensor(v, device=dev),
+              triangles=torch.as_tensor(t, device=dev),
+              colors=torch.as_tensor(colors, device=dev))
+  target_mesh = dict(vertices=torch.as_tensor(v_target, device=dev),
+                     triangles=mesh["triangles"], colors=mesh["colors"])
+  cams_t = [torch.as_tensor(c, device=dev) for c in cams]
+  return dict(v=v, t=t, colors=colors, cams=cams, mesh=mesh,
+              target_mesh=target_mesh, cams_t=cams_t)
+
+
+def time_stage(fn, reps):
+  import torch
+  start = torch.cuda.Event(enable_timing=True)
+  end = torch.cuda.Event(enable_timing=True)
+  fn()
+  start.record()
+  for _ in range(reps):
+    fn()
+  end.record()
+  torch.cuda.synchronize()
+  return start.elapsed_time(end) / reps  # ms
+
+
+def plain_step(w, args):
+  import torch
+  from diffren_amd import render
+  vv = w["mesh"]["vertices"].clone().requires_grad_(True)
+  cc = w["mesh"]["colors"].clone().requires_grad_(True)
+  npix = args.views * args.resolution ** 2 * 4
+  for i in range(args.views):
+    img = render.render_triangles(vv, {"c": cc}, w["mesh"]["triangles"],
+                                  w["cams_t"][i], args.resolution,
+                                  args.resolution, lambda a: a["c"],
+                                  num_layers=args.layers)
+    loss = ((img - w["targets"][i]) ** 2).sum() / npix
+    loss.backward()
+  return vv.grad, cc.grad
+
+
+def cpu_baseline(w, args):
+  """Reference CPU rasterizer + float32 CPU interpolate/splat fwd+bwd."""
+  import numpy as np
+  import torch
+  from oracle import raster_oracle
+  from oracle import splat_oracle as so
+  kind = "reference" if raster_oracle.ref_available() else "port"
+  ras = raster_oracle.ref_rasterize if kind == "reference" else \
+      raster_oracle.rasterize
+  R, L = args.resolution, args.layers
+  tgt = w["targets"][0].cpu()
+  t0 = time.perf_counter()
+  clip = so.transform(w["cams"][0], w["v"], torch.float32)
+  ids, z, b = ras(clip.numpy(), w["t"], R, R, L, 1)
+  t1 = time.perf_counter()
+  v32 = torch.as_tensor(w["v"]).requires_grad_(True)
+  c32 = torch.as_tensor(w["colors"]).requires_grad_(True)
+  out = so.render_splat(v32, c32, w["cams"][0], ids, b, w["t"])
+  loss = ((out - tgt) ** 2).sum() / (R * R * 4)
+  loss.backward()
+  t2 = time.perf_counter()
+  return dict(value=1.0 / (t2 - t0), unit="renders/s", cores=int(
+      torch.get_num_threads()), kind=kind,
+              sample=(f"1 view of the workload, fwd+bwd: {kind} CPU "
+                      f"rasterizer single-threaded ({(t1 - t0) * 1e3:.0f} ms)"
+                      f" + float32 torch-CPU interpolate/splat/over fwd+bwd "
+                      f"on {torch.get_num_threads()} threads "
+                      f"({(t2 - t1) * 1e3:.0f} ms)"),
+              raster_ms=(t1 - t0) * 1e3, splat_ms=(t2 - t1) * 1e3)
+
+
+def grad_check(w, args, dev):
+  """Vertex/colour gradient of view 0: fused GPU path vs float64 oracle."""
+  import numpy as np
+  import torch
+  from diffren_amd import batch
+  from oracle import raster_oracle
+  from oracle import splat_oracle as so
+  R, L = args.resolution, args.layers
+  r = batch.BatchedSplatRenderer([w["mesh"]], [(0, w["cams_t"][0], R, R)], L,
+                                 loss_scale=1.0 / (R * R * 4))
+  r.targets[0].copy_(w["targets"][0])
+  r.step()
+  torch.cuda.synchronize()
+  ids = r.ids[0].cpu().numpy()
+  b = r.barys[0].cpu().numpy()
+  clip32 = so.transform(w["cams"][0], w["v"], torch.float32).numpy()
+  ids_o, _, b_o = raster_oracle.rasterize(clip32, w["t"], R, R, L, 1)
+  ids_exact = bool(np.array_equal(ids, ids_o) and np.array_equal(
+      b.view(np.uint32), b_o.view(np.uint32)))
+  v64 = torch.as_tensor(w["v"], dtype=torch.float64).requires_grad_(True)
+  c64 = torch.as_tensor(w["colors"], dtype=torch.float64).requires_grad_(True)
+  out = so.render_splat(v64, c64, w["cams"][0], ids_o, b_o, w["t"])
+  loss = ((out - w["targets"][0].cpu().double()) ** 2).sum() / (R * R * 4)
+  loss.backward()
+
+  def rel(a, b):
+    a = a.cpu().double()
+    return float((a - b).abs().max() / b.abs().max())
+
+  return dict(vertex_grad_rel_err=rel(r.d_vertices[0], v64.grad),
+              color_grad_rel_err=rel(r.d_colors[0], c64.grad),
+              loss_rel_err=abs(r.loss.item() - loss.item()) / abs(loss.item()),
+              triangle_ids_bit_exact=ids_exact)
+
+
+def main():
+  args = parse()
+  if args.gpus != 1:
+    print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not "
+          "supported (views and meshes would shard across GPUs with one "
+          "gradient all-reduce, which this build does not implement)",
+          file=sys.stderr)
+    sys.exit(2)
+  import numpy as np
+  import torch
+  from diffren_amd import batch
+  from diffren_amd import rasterize
+
+  dev = torch.device("cuda:0")
+  log("building workload")
+  w = build_workload(args, dev)
+  R, L, B = args.resolution, args.layers, args.views
+  T = w["t"].shape[0]
+  V = w["v"].shape[0]
+  C = 4
+  jobs = [(0, w["cams_t"][i], R, R) for i in range(B)]
+
+  # Targets: forward renders of the perturbed mesh.
+  tr = batch.BatchedSplatRenderer([w["target_mesh"]], jobs, L)
+  tr.render()
+  torch.cuda.synchronize()
+  w["targets"] = [img.clone() for img in tr.images]
+  del tr
+
+  r = batch.BatchedSplatRenderer([w["mesh"]], jobs, L)
+  for i in range(B):
+    r.targets[i].copy_(w["targets"][i])
+  entries, overflow = (r.step(), r.bin_stats())[1]
+  log(f"bin entries {entries} overflow {overflow}")
+
+  use_graph = not args.no_graph
+  if use_graph:
+    r.capture()
+    run = r.replay
+  else:
+    run = r.step
+  for _ in range(args.warmup):
+    run()
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(args.steps):
+    run()
+  torch.cuda.synchronize()
+  dt = (time.perf_counter() - t0) / args.steps
+  renders_per_s = B / dt
+  log(f"batched fused: {dt * 1e3:.3f} ms/step, {renders_per_s:.1f} renders/s")
+
+  # Per-stage HIP-event times (eager, same stream as the kernels).
+  reps = 10
+  stage_ms = dict(
+      transform=time_stage(r.transform, reps),
+      rasterize=time_stage(r.rasterize, reps),
+      splat_target=time_stage(lambda: r.splat_forward(1), reps),
+      splat_source=time_stage(r.splat_backward, reps))
+  log("stage ms (all views): " + json.dumps(stage_ms))
+
+  # Algorithmic bytes per launch (all views), fp32/i32, each API-level input
+  # and output counted once.
+  P = R * R
+  lp = L * P
+  bytes_target = B * (lp * (4 + 12 + 4) + T * 12 + V * 16 + V * 4 * C +
+                      P * 4 * C + P * 4 * C + P * 3 * (C + 1) * 4)
+  bytes_source = B * (lp * (4 + 12 + 4) + T * 12 + V * 16 + V * 4 * C +
+                      P * 3 * (C + 1) * 4 + 16 * 4) + V * 4 * (C + 3)
+  bytes_raster = B * (T * 12 + V * 16 + lp * (4 + 4 + 12))
+  per_stage = {
+      "splat_target": ("render_splat target pass (interpolate+splat+over+L2"
+                       " grad)", bytes_target),
+      "splat_source": ("render_splat source pass (splat/interpolate/"
+                       "transform VJP)", bytes_source),
+      "rasterize": ("tile rasterizer (bin + tile raster + resolve)",
+                    bytes_raster),
+  }
+  dom = max(per_stage, key=lambda k: stage_ms[k])
+  name, nbytes = per_stage[dom]
+  achieved = nbytes / (stage_ms[dom] * 1e-3) / 1e9
+  roofline = dict(bound="hbm", kernel=name, achieved=round(achieved, 1),
+                  peak=HBM_PEAK_GBS, unit="GB/s",
+                  frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                  algorithmic_bytes=nbytes, avg_ms=round(stage_ms[dom], 4))
+
+  # Plain path: one render_triangles per view + autograd.
+  for _ in range(1):
+    plain_step(w, args)
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for _ in range(args.plain_steps):
+    plain_step(w, args)
+  torch.cuda.synchronize()
+  plain_dt = (time.perf_counter() - t0) / args.plain_steps
+  plain_rps = B / plain_dt
+  log(f"plain: {plain_dt * 1e3:.3f} ms/step, {plain_rps:.1f} renders/s")
+
+  extra = {}
+  if not args.no_grad_check:
+    log("float64 gradient check (view 0)")
+    extra.update(grad_check(w, args, dev))
+    log(json.dumps(extra))
+  cpu = None
+  if not args.no_cpu_baseline:
+    log("CPU baseline (view 0)")
+    cpu = cpu_baseline(w, args)
+    log(json.dumps(cpu))
+
+  out = {
+      "metric": ("fwd+bwd renders/s @1M tris 1024² (batched vs plain "
+                 "GPU); Mtri/s; grad rel err"),
+      "value": round(renders_per_s, 2),
+      "unit": "renders/s",
+      "n_gpus": 1,
+      "steps": args.steps,
+      "warmup": args.warmup,
+      "ms_per_step": round(dt * 1e3, 4),
+      "higher_is_better": True,
+      "scaling": "weak",
+      "vs_baseline": None,
+      "dtype": "f32",
+      "data": "synthetic (seeded Perlin-displaced UV sphere, seeded cameras)",
+      "config": {"workload": "1M-triangle sphere, 1024x1024, 3 layers, 8 "
+                             "views, RGBA, fwd + L2 loss + bwd, batched "
+                             "fused path" +
+                             (" in a HIP graph" if use_graph else ""),
+                 "triangles": T, "vertices": V, "resolution": R, "layers": L,
+                 "views": B, "channels": C, "parallelism": "single GPU"},
+      "mtri_per_s": round(renders_per_s * T / 1e6, 1),
+      "plain_renders_per_s": round(plain_rps, 2),
+      "speedup_vs_plain": round(renders_per_s / plain_rps, 2),
+      "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+      "roofline": roofline,
+      "cpu_baseline": cpu,
+  }
+  out.update(extra)
+  print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+  main()

@@ -32,6 +32,26 @@ class RasterJob(ctypes.Structure):
               ("tile_begin", _c_i64)]
 
 
+class PhongParams(ctypes.Structure):
+  _fields_ = [("ka", ctypes.c_float), ("kd", ctypes.c_float),
+              ("ks", ctypes.c_float), ("ns", ctypes.c_float),
+              ("attenuation_constant", ctypes.c_float),
+              ("attenuation_linear", ctypes.c_float),
+              ("attenuation_quadratic", ctypes.c_float)]
+
+
+class SplatJob(ctypes.Structure):
+  _fields_ = [("clip_vertices", _c_ptr), ("triangles", _c_ptr),
+              ("colors", _c_ptr), ("camera", _c_ptr),
+              ("triangle_ids", _c_ptr), ("z_buffer", _c_ptr),
+              ("barycentrics", _c_ptr), ("image", _c_ptr),
+              ("target", _c_ptr), ("d_image", _c_ptr),
+              ("grad_scratch", _c_ptr), ("d_colors", _c_ptr),
+              ("d_vertices", _c_ptr), ("num_vertices", _c_int),
+              ("num_triangles", _c_int), ("width", _c_int),
+              ("height", _c_int), ("tile_begin", _c_i64)]
+
+
 # name -> (restype, argtypes).  Keep in the order of include/diffren_amd.h.
 SIGNATURES = {
     "dr_last_error": (ctypes.c_char_p, []),
@@ -64,7 +84,34 @@ SIGNATURES = {
     "dr_splat_bwd": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_int, _c_int, _c_int,
                               _c_int, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_size,
                               _c_ptr]),
+    "dr_gbuffer": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_int, _c_ptr, _c_ptr,
+                            _c_ptr, _c_int, _c_ptr, _c_i64, _c_ptr, _c_ptr]),
+    "dr_phong_shade": (_c_int, [_c_ptr, _c_i64, _c_ptr, _c_int, _c_ptr,
+                                _c_ptr, _c_ptr]),
+    "dr_mlp_num_params": (_c_int, []),
+    "dr_mlp_packed_bytes": (_c_size, []),
+    "dr_mlp_pack": (_c_int, [_c_ptr, _c_ptr, _c_ptr]),
+    "dr_mlp_forward": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_i64, _c_ptr,
+                                _c_int, _c_ptr]),
+    "dr_mlp_train_step": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
+                                   _c_i64, ctypes.c_float, _c_ptr, _c_ptr,
+                                   _c_int, _c_ptr]),
+    "dr_mlp_adam": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_int,
+                             ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                             ctypes.c_float, _c_ptr]),
+    "dr_splat_jobs_prepare": (_c_int, [_c_ptr, _c_int, _c_int, _c_ptr]),
+    "dr_render_splat_forward": (_c_int, [_c_ptr, _c_int, _c_i64, _c_int,
+                                         _c_int, _c_int, ctypes.c_float,
+                                         _c_ptr, _c_ptr]),
+    "dr_render_splat_backward": (_c_int, [_c_ptr, _c_int, _c_i64, _c_int,
+                                          _c_int, _c_ptr]),
 }
+
+
+def table_to_device(table, device):
+  """Copies a ctypes array of job structs into a device byte tensor."""
+  host = torch.frombuffer(bytearray(bytes(table)), dtype=torch.uint8)
+  return host.to(device)
 
 _lib = None
 

@@ -89,9 +89,9 @@ __global__ __launch_bounds__(kBlock) void interp_bwd_kernel(
   for (int c = 0; c < C; ++c) {
     const float gc = g[c];
     if (gc == 0.0f) continue;
-    atomicAdd(d0 + c, b0 * gc);
-    atomicAdd(d1 + c, b1 * gc);
-    atomicAdd(d2 + c, b2 * gc);
+    unsafeAtomicAdd(d0 + c, b0 * gc);
+    unsafeAtomicAdd(d1 + c, b1 * gc);
+    unsafeAtomicAdd(d2 + c, b2 * gc);
   }
 }
 

@@ -1,0 +1,521 @@
+// Fused, batched rasterize-then-splat render for gfx950: forward, L2 loss
+// gradient and backward over many (mesh, camera, resolution) jobs per launch.
+//
+// Per job the inputs are the rasterizer's outputs (triangle ids, z, and
+// barycentrics, layer-major [L][H][W]), the clip-space vertices, the
+// triangles and per-vertex colours; the shading function is the identity on
+// the interpolated colours (render.py:123-132 with shading_function =
+// lambda a: a["colors"]).  What the reference computes as separate XLA passes
+// -- interpolate the colours (interpolate.py:24-63), interpolate clip xyzw
+// with background (0,0,1,1) and map to the viewport (splat.py:209-221),
+// splat_at_pixel_centers (splat.py:26-175) -- is done here without writing any
+// intermediate frame:
+//
+//  target pass (one block per 64x4 tile of output pixels of one job):
+//    stage the tile plus a 1-pixel halo, all layers, in LDS: fg mask,
+//    viewport x, y, z and the interpolated colour, computed from ids,
+//    barycentrics and vertex gathers; then pull-splat every output pixel from
+//    its 3x3 neighbourhood (same arithmetic and order as splat.hip), write the
+//    image and, when a gradient is wanted, back-propagate d image (given, or
+//    2 (image - target) * loss_scale for an L2 loss) through `over` and the
+//    normalisation into G[pixel][class] = (d acc colour, d acc weight).
+//  source pass (one thread per source pixel, all layers): re-interpolate the
+//    pixel's colour and position, pull G from the 9 pixels it splats onto,
+//    form d colour and d viewport xy, map d xy to d clip (x, y, w), and
+//    scatter b_k-weighted gradients to the triangle's three vertices --
+//    d colours[v] and d vertices[v] = M^T d clip (or d clip itself when no
+//    camera is given).  Float atomics: summation order is not fixed.
+//
+// Viewport z is taken from the rasterizer's z buffer, ((z + 1) * 1) * 0.5:
+// the rasterizer computes z = ((b0 z0 + b1 z1) + b2 z2) / ((b0 w0 + b1 w1) +
+// b2 w2) (rasterize_triangles_impl.cc:87-89), the same expression the
+// interpolated clip buffer divides out (splat.py:218), so it is the same
+// value bit for bit.
+#pragma clang fp contract(off)
+
+#include <cmath>
+
+#include "dr_common.h"
+#include "../../include/diffren_amd.h"
+
+namespace dr {
+namespace {
+
+constexpr int kBX = 64, kBY = 4;
+constexpr int kRX = kBX + 2, kRY = kBY + 2, kRegion = kRX * kRY;
+constexpr int kMaxLayers = 8;
+constexpr int kMaxChannels = 16;
+
+__device__ __forceinline__ int find_job(const dr_splat_job* jobs, int n,
+                                        int64_t tile) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].tile_begin <= tile) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+struct Corner {
+  int v[3];
+  float b[3];
+};
+
+// Triangle corners of pixel-layer q; false on background or bad indices.
+__device__ __forceinline__ bool corners(const dr_splat_job& J, int64_t q,
+                                        Corner& k) {
+  const int t = J.triangle_ids[q];
+  if (t < 0 || t >= J.num_triangles) return false;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    k.v[i] = J.triangles[3 * int64_t(t) + i];
+    if (unsigned(k.v[i]) >= unsigned(J.num_vertices)) return false;
+    k.b[i] = J.barycentrics[3 * q + i];
+  }
+  return true;
+}
+
+__device__ __forceinline__ float interp3(const Corner& k, float a0, float a1,
+                                         float a2) {
+  return (k.b[0] * a0 + k.b[1] * a1) + k.b[2] * a2;
+}
+
+// Viewport x, y of pixel-layer q (splat.py:214-221).
+__device__ __forceinline__ void viewport_xy(const dr_splat_job& J,
+                                            const Corner& k, float& x,
+                                            float& y, float& cx, float& cy,
+                                            float& cw) {
+  const float4* cv = reinterpret_cast<const float4*>(J.clip_vertices);
+  const float4 p0 = cv[k.v[0]], p1 = cv[k.v[1]], p2 = cv[k.v[2]];
+  cx = interp3(k, p0.x, p1.x, p2.x);
+  cy = interp3(k, p0.y, p1.y, p2.y);
+  cw = interp3(k, p0.w, p1.w, p2.w);
+  x = ((cx / cw + 1.0f) * float(J.width)) * 0.5f;
+  y = ((cy / cw + 1.0f) * float(J.height)) * 0.5f;
+}
+
+__device__ __forceinline__ float viewport_z(float z) {
+  return ((z + 1.0f) * 1.0f) * 0.5f;
+}
+
+__device__ __forceinline__ int surface_index(const float* zS, const float* zT,
+                                             int L) {
+  float bc = fabsf(zS[0] - zT[0]);
+  int ic = 0;
+  float bt = fabsf(zT[0] - zS[0]);
+  int it = 0;
+  for (int l = 1; l < L; ++l) {
+    const float dc = fabsf(zS[l] - zT[0]);
+    if (dc < bc) { bc = dc; ic = l; }
+    const float dt = fabsf(zT[l] - zS[0]);
+    if (dt < bt) { bt = dt; it = l; }
+  }
+  return bt < bc ? -it : ic;
+}
+
+__device__ __forceinline__ int layer_class(int surface, int l) {
+  return surface > l ? 0 : (surface == l ? 1 : 2);
+}
+
+__device__ __forceinline__ float tap_weight(float coef, float sx, float sy,
+                                            int u, int v, float& dx,
+                                            float& dy) {
+  dx = ((floorf(sx) + 0.5f) - sx) + float(u);
+  dy = ((floorf(sy) + 0.5f) - sy) + float(v);
+  const float d2 = dx * dx + dy * dy;
+  return (1.0f * coef) * expf(-2.0f * d2);
+}
+
+__device__ __forceinline__ float norm_scale(float w) {
+  return 1.0f / (fmaxf(w - 1.0f, 0.0f) + 1.0f);
+}
+
+// LDS stage, structure of arrays over [L][kRegion]: x, y, z, colour[C].
+// Background / outside-the-image entries have x = NaN as the mask (the fg
+// mask is 1 exactly where the rasterizer stored a triangle).
+template <int C>
+__global__ __launch_bounds__(kBX* kBY) void target_kernel(
+    const dr_splat_job* __restrict__ jobs, int num_jobs, int L, float coef,
+    int grad_mode, float loss_scale, float* __restrict__ loss) {
+  extern __shared__ float lds[];
+  const int R = kRegion;
+  float* sx = lds;
+  float* sy = sx + L * R;
+  float* sz = sy + L * R;
+  float* sc = sz + L * R;  // [L][R][C]
+
+  const int64_t tile = blockIdx.x;
+  const dr_splat_job& J = jobs[find_job(jobs, num_jobs, tile)];
+  const int W = J.width, H = J.height;
+  const int ntx = (W + kBX - 1) / kBX;
+  const int local = int(tile - J.tile_begin);
+  const int x0 = (local % ntx) * kBX, y0 = (local / ntx) * kBY;
+  const int64_t P = int64_t(W) * H;
+  const int tid = threadIdx.y * kBX + threadIdx.x;
+
+  for (int i = tid; i < L * R; i += kBX * kBY) {
+    const int l = i / R, r = i % R;
+    const int px = x0 - 1 + r % kRX, py = y0 - 1 + r / kRX;
+    float x = __int_as_float(0x7fc00000), y = 0.0f, z = 1.0f;
+    float col[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) col[c] = 0.0f;
+    if (px >= 0 && px < W && py >= 0 && py < H) {
+      const int64_t q = l * P + int64_t(py) * W + px;
+      z = viewport_z(J.z_buffer[q]);
+      Corner k;
+      if (corners(J, q, k)) {
+        float cx, cy, cw;
+        viewport_xy(J, k, x, y, cx, cy, cw);
+        const float* a0 = J.colors + int64_t(k.v[0]) * C;
+        const float* a1 = J.colors + int64_t(k.v[1]) * C;
+        const float* a2 = J.colors + int64_t(k.v[2]) * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c) col[c] = interp3(k, a0[c], a1[c], a2[c]);
+      }
+    }
+    sx[i] = x;
+    sy[i] = y;
+    sz[i] = z;
+#pragma unroll
+    for (int c = 0; c < C; ++c) sc[i * C + c] = col[c];
+  }
+  __syncthreads();
+
+  const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
+  float partial = 0.0f;
+  if (x < W && y < H) {
+    const int rT = (threadIdx.y + 1) * kRX + threadIdx.x + 1;
+    float zT[kMaxLayers], zS[kMaxLayers];
+    for (int l = 0; l < L; ++l) zT[l] = sz[l * R + rT];
+    int surf[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int u = t / 3 - 1, v = t % 3 - 1;
+      const int rS = rT - v * kRX - u;
+      for (int l = 0; l < L; ++l) zS[l] = sz[l * R + rS];
+      surf[t] = surface_index(zS, zT, L);
+    }
+    float acc[3][C + 1];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int c = 0; c <= C; ++c) acc[k][c] = 0.0f;
+    for (int l = 0; l < L; ++l) {
+#pragma unroll
+      for (int t = 0; t < 9; ++t) {
+        const int u = t / 3 - 1, v = t % 3 - 1;
+        const int i = l * R + rT - v * kRX - u;
+        const float px = sx[i];
+        if (px != px) continue;  // background or outside the image
+        float dx, dy;
+        const float w = tap_weight(coef, px, sy[i], u, v, dx, dy);
+        const int k = layer_class(surf[t], l);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const float wc = w * sc[i * C + c];
+          acc[0][c] += k == 0 ? wc : 0.0f;
+          acc[1][c] += k == 1 ? wc : 0.0f;
+          acc[2][c] += k == 2 ? wc : 0.0f;
+        }
+        acc[0][C] += k == 0 ? w : 0.0f;
+        acc[1][C] += k == 1 ? w : 0.0f;
+        acc[2][C] += k == 2 ? w : 0.0f;
+      }
+    }
+    float s[3], n[3][C];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      s[k] = norm_scale(acc[k][C]);
+#pragma unroll
+      for (int c = 0; c < C; ++c) n[k][c] = acc[k][c] * s[k];
+    }
+    const float a0 = n[0][C - 1], a1 = n[1][C - 1];
+    const int64_t p = int64_t(y) * W + x;
+    float g[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      float o = n[2][c];
+      o = n[1][c] + (1.0f - a1) * o;
+      o = n[0][c] + (1.0f - a0) * o;
+      if (J.image) J.image[p * C + c] = o;
+      if (grad_mode == 1) {
+        const float d = o - J.target[p * C + c];
+        partial += d * d;
+        g[c] = (2.0f * d) * loss_scale;
+      } else if (grad_mode == 2) {
+        g[c] = J.d_image[p * C + c];
+      }
+    }
+    if (grad_mode != 0) {
+      float dn[3][C];
+      float da0 = 0.0f, da1 = 0.0f;
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const float o1 = n[1][c] + (1.0f - a1) * n[2][c];
+        dn[0][c] = g[c];
+        da0 -= g[c] * o1;
+        const float go1 = (1.0f - a0) * g[c];
+        dn[1][c] = go1;
+        da1 -= go1 * n[2][c];
+        dn[2][c] = (1.0f - a1) * go1;
+      }
+      dn[0][C - 1] += da0;
+      dn[1][C - 1] += da1;
+      float* gp = J.grad_scratch + p * 3 * (C + 1);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float ds = acc[k][C] > 1.0f ? -(s[k] * s[k]) : 0.0f;
+        float dw = 0.0f;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          gp[k * (C + 1) + c] = dn[k][c] * s[k];
+          dw += dn[k][c] * acc[k][c];
+        }
+        gp[k * (C + 1) + C] = dw * ds;
+      }
+    }
+  }
+  if (grad_mode == 1 && loss) {
+    // Wave reduction, then one atomic per wave.
+    for (int off = 32; off > 0; off >>= 1) partial += __shfl_xor(partial, off);
+    if (__lane_id() == 0 && partial != 0.0f) unsafeAtomicAdd(loss, partial);
+  }
+}
+
+template <int C>
+__global__ __launch_bounds__(kBX* kBY) void source_kernel(
+    const dr_splat_job* __restrict__ jobs, int num_jobs, int L, float coef) {
+  const int64_t tile = blockIdx.x;
+  const dr_splat_job& J = jobs[find_job(jobs, num_jobs, tile)];
+  const int W = J.width, H = J.height;
+  const int ntx = (W + kBX - 1) / kBX;
+  const int local = int(tile - J.tile_begin);
+  const int x = (local % ntx) * kBX + threadIdx.x;
+  const int y = (local / ntx) * kBY + threadIdx.y;
+  if (x >= W || y >= H) return;
+  const int64_t P = int64_t(W) * H;
+  const int64_t p = int64_t(y) * W + x;
+
+  // Skip pixels with no foreground layer before touching the neighbours.
+  bool any = false;
+  for (int l = 0; l < L; ++l) any |= J.triangle_ids[l * P + p] >= 0;
+  if (!any) return;
+
+  float zS[kMaxLayers], zT[kMaxLayers];
+  for (int l = 0; l < L; ++l) zS[l] = viewport_z(J.z_buffer[l * P + p]);
+  int surf[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    const int u = t / 3 - 1, v = t % 3 - 1;
+    const int tx = x + u, ty = y + v;
+    surf[t] = 0;
+    if (tx < 0 || tx >= W || ty < 0 || ty >= H) continue;
+    const int64_t pt = int64_t(ty) * W + tx;
+    for (int l = 0; l < L; ++l) zT[l] = viewport_z(J.z_buffer[l * P + pt]);
+    surf[t] = surface_index(zS, zT, L);
+  }
+  for (int l = 0; l < L; ++l) {
+    const int64_t q = l * P + p;
+    Corner k;
+    if (!corners(J, q, k)) continue;
+    float sx, sy, cx, cy, cw;
+    viewport_xy(J, k, sx, sy, cx, cy, cw);
+    const float* a0 = J.colors + int64_t(k.v[0]) * C;
+    const float* a1 = J.colors + int64_t(k.v[1]) * C;
+    const float* a2 = J.colors + int64_t(k.v[2]) * C;
+    float col[C], dcol[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      col[c] = interp3(k, a0[c], a1[c], a2[c]);
+      dcol[c] = 0.0f;
+    }
+    float gx = 0.0f, gy = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      const int u = t / 3 - 1, v = t % 3 - 1;
+      const int tx = x + u, ty = y + v;
+      if (tx < 0 || tx >= W || ty < 0 || ty >= H) continue;
+      float dx, dy;
+      const float w = tap_weight(coef, sx, sy, u, v, dx, dy);
+      const int cls = layer_class(surf[t], l);
+      const float* gp =
+          J.grad_scratch + ((int64_t(ty) * W + tx) * 3 + cls) * (C + 1);
+      float dw = gp[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        dcol[c] += w * gp[c];
+        dw += gp[c] * col[c];
+      }
+      const float dw4 = dw * 4.0f * w;
+      gx += dw4 * dx;
+      gy += dw4 * dy;
+    }
+    // x = ((cx / cw + 1) W) / 2  ->  d cx = gx W / (2 cw),
+    // d cw = -(gx W cx + gy H cy) / (2 cw^2).
+    const float hx = gx * (0.5f * float(W)), hy = gy * (0.5f * float(H));
+    const float inv_w = 1.0f / cw;
+    const float dcx = hx * inv_w, dcy = hy * inv_w;
+    const float dcw = -(dcx * cx + dcy * cy) * inv_w;
+    if (J.d_colors) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        float* d = J.d_colors + int64_t(k.v[i]) * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+          if (dcol[c] != 0.0f) unsafeAtomicAdd(d + c, k.b[i] * dcol[c]);
+      }
+    }
+    if (J.d_vertices) {
+      if (J.camera) {
+        // d v_j = sum_i M[i][j] d clip_i over i in {x, y, w} (z gets none).
+        const float* M = J.camera;
+        float dv[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          dv[j] = (M[j] * dcx + M[4 + j] * dcy) + M[12 + j] * dcw;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          float* d = J.d_vertices + int64_t(k.v[i]) * 3;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) unsafeAtomicAdd(d + j, k.b[i] * dv[j]);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          float* d = J.d_vertices + int64_t(k.v[i]) * 4;
+          unsafeAtomicAdd(d + 0, k.b[i] * dcx);
+          unsafeAtomicAdd(d + 1, k.b[i] * dcy);
+          unsafeAtomicAdd(d + 3, k.b[i] * dcw);
+        }
+      }
+    }
+  }
+}
+
+size_t target_lds_bytes(int L, int C) {
+  return size_t(L) * kRegion * (3 + C) * sizeof(float);
+}
+
+float splat_coef() {
+  double sum = 0.0;
+  for (int u = -1; u <= 1; ++u)
+    for (int v = -1; v <= 1; ++v) sum += std::exp(-2.0 * (u * u + v * v));
+  return float(1.05 / sum);
+}
+
+template <int C>
+int launch_target(const dr_splat_job* jobs, int n, int64_t tiles, int L,
+                  int grad_mode, float loss_scale, float* loss,
+                  hipStream_t s) {
+  const size_t lds = target_lds_bytes(L, C);
+  if (lds > 65536) {
+    DR_CHECK_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&target_kernel<C>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  }
+  target_kernel<C><<<unsigned(tiles), dim3(kBX, kBY), lds, s>>>(
+      jobs, n, L, splat_coef(), grad_mode, loss_scale, loss);
+  DR_CHECK_LAUNCH("render splat target pass");
+  return kOk;
+}
+
+template <int C>
+int launch_source(const dr_splat_job* jobs, int n, int64_t tiles, int L,
+                  hipStream_t s) {
+  source_kernel<C><<<unsigned(tiles), dim3(kBX, kBY), 0, s>>>(jobs, n, L,
+                                                              splat_coef());
+  DR_CHECK_LAUNCH("render splat source pass");
+  return kOk;
+}
+
+#define DR_RENDER_SWITCH(C, CALL) \
+  switch (C) {                    \
+    case 1: return CALL(1);       \
+    case 2: return CALL(2);       \
+    case 3: return CALL(3);       \
+    case 4: return CALL(4);       \
+    case 5: return CALL(5);       \
+    case 6: return CALL(6);       \
+    case 7: return CALL(7);       \
+    case 8: return CALL(8);       \
+    case 12: return CALL(12);     \
+    case 16: return CALL(16);     \
+  }
+
+int check(int n, int64_t tiles, int L, int C) {
+  DR_CHECK_ARG(n > 0, "need at least one job");
+  DR_CHECK_ARG(tiles > 0 && tiles < (int64_t(1) << 31), "bad tile count");
+  DR_CHECK_ARG(L >= 1 && L <= kMaxLayers, "num_layers must be in [1, 8]");
+  DR_CHECK_ARG(C >= 1 && C <= kMaxChannels && (C <= 8 || C == 12 || C == 16),
+               "fused render supports 1-8, 12 or 16 channels");
+  DR_CHECK_ARG(target_lds_bytes(L, C) <= 160 * 1024,
+               "layers x channels exceed the LDS stage");
+  return kOk;
+}
+
+}  // namespace
+}  // namespace dr
+
+extern "C" {
+
+int dr_splat_jobs_prepare(dr_splat_job* jobs, int32_t n, int32_t channels,
+                          int64_t* total_tiles) {
+  DR_CHECK_ARG(jobs != nullptr && n > 0, "need at least one job");
+  DR_CHECK_ARG(channels >= 1 && channels <= dr::kMaxChannels,
+               "bad channel count");
+  int64_t tiles = 0;
+  for (int i = 0; i < n; ++i) {
+    dr_splat_job& j = jobs[i];
+    DR_CHECK_ARG(j.width > 0 && j.height > 0, "image size must be > 0");
+    DR_CHECK_ARG(j.num_vertices >= 0 && j.num_triangles >= 0,
+                 "negative mesh size");
+    DR_CHECK_ARG(j.triangle_ids && j.z_buffer && j.barycentrics,
+                 "null rasterizer output");
+    DR_CHECK_ARG(j.num_triangles == 0 ||
+                     (j.clip_vertices && j.triangles && j.colors),
+                 "null mesh pointer");
+    DR_CHECK_ARG((reinterpret_cast<uintptr_t>(j.clip_vertices) & 15) == 0,
+                 "clip_vertices must be 16-byte aligned");
+    j.tile_begin = tiles;
+    tiles += int64_t((j.width + dr::kBX - 1) / dr::kBX) *
+             ((j.height + dr::kBY - 1) / dr::kBY);
+  }
+  if (total_tiles) *total_tiles = tiles;
+  return DR_OK;
+}
+
+int dr_render_splat_forward(const dr_splat_job* jobs_device, int32_t num_jobs,
+                            int64_t total_tiles, int32_t num_layers,
+                            int32_t channels, int32_t grad_mode,
+                            float loss_scale, float* loss,
+                            dr_stream_t stream) {
+  int rc = dr::check(num_jobs, total_tiles, num_layers, channels);
+  if (rc != DR_OK) return rc;
+  DR_CHECK_ARG(jobs_device != nullptr, "jobs is null");
+  DR_CHECK_ARG(grad_mode >= 0 && grad_mode <= 2, "grad_mode must be 0, 1, 2");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define DR_CALL(CC)                                                     \
+  dr::launch_target<CC>(jobs_device, num_jobs, total_tiles, num_layers, \
+                        grad_mode, loss_scale, loss, s)
+  DR_RENDER_SWITCH(channels, DR_CALL)
+#undef DR_CALL
+  return dr::fail(dr::kUnsupported, "unsupported channel count");
+}
+
+int dr_render_splat_backward(const dr_splat_job* jobs_device,
+                             int32_t num_jobs, int64_t total_tiles,
+                             int32_t num_layers, int32_t channels,
+                             dr_stream_t stream) {
+  int rc = dr::check(num_jobs, total_tiles, num_layers, channels);
+  if (rc != DR_OK) return rc;
+  DR_CHECK_ARG(jobs_device != nullptr, "jobs is null");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define DR_CALL(CC) \
+  dr::launch_source<CC>(jobs_device, num_jobs, total_tiles, num_layers, s)
+  DR_RENDER_SWITCH(channels, DR_CALL)
+#undef DR_CALL
+  return dr::fail(dr::kUnsupported, "unsupported channel count");
+}
+
+}  // extern "C"

@@ -1,0 +1,152 @@
+"""Deferred shading on the GPU: G-buffers, many-light Phong and a learned
+MLP shader with its training loop.
+
+gbuffer(...)          -> [L,H,W,12] position, normal, view direction, albedo
+phong(...)            -> [...,3] many-light form of shaders.point_light
+                         (diffren/jax/utils/shaders.py:100-173), ambient once
+MLPShader             12 -> 64x4 -> 3 MLP, bfloat16 MFMA, float32 accumulation,
+                      trained by the library's own forward/backward/Adam
+                      kernels; `shade_with_fallback` keeps the Phong result
+                      wherever the network's per-pixel error exceeds a bound.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from diffren_amd import _native
+
+PHONG_DEFAULTS = dict(ka=0.3, kd=1.4, ks=0.5, ns=16.0,
+                      attenuation_constant=1.0, attenuation_linear=0.7,
+                      attenuation_quadratic=1.8)
+
+
+def gbuffer(framebuffer, positions, normals, albedo, eye):
+  """[L,H,W,12] G-buffer from a rasterized Framebuffer and [V,3] attributes."""
+  ids = framebuffer.triangle_id[..., 0].contiguous()
+  b = framebuffer.barycentrics.contiguous()
+  tris = framebuffer.triangles
+  pos = positions.to(torch.float32).contiguous()
+  nrm = normals.to(torch.float32).contiguous()
+  alb = albedo.to(torch.float32).contiguous()
+  eye_t = torch.as_tensor(eye, dtype=torch.float32,
+                          device=pos.device).contiguous()
+  out = torch.empty(tuple(ids.shape) + (12,), dtype=torch.float32,
+                    device=pos.device)
+  _native.check(
+      _native.lib().dr_gbuffer(
+          _native.ptr(ids), _native.ptr(b), _native.ptr(tris), tris.shape[0],
+          _native.ptr(pos), _native.ptr(nrm), _native.ptr(alb), pos.shape[0],
+          _native.ptr(eye_t), ids.numel(), _native.ptr(out),
+          _native.stream_handle()), "dr_gbuffer")
+  return out
+
+
+def phong(gbuf, lights, **params):
+  """gbuf [...,12], lights [K,4] (xyz, intensity) -> [...,3]."""
+  prm = dict(PHONG_DEFAULTS)
+  prm.update(params)
+  g = gbuf.contiguous()
+  lt = lights.to(torch.float32).contiguous()
+  out = torch.empty(tuple(g.shape[:-1]) + (3,), dtype=torch.float32,
+                    device=g.device)
+  cp = _native.PhongParams(**{k: float(v) for k, v in prm.items()})
+  _native.check(
+      _native.lib().dr_phong_shade(_native.ptr(g), g.numel() // 12,
+                                   _native.ptr(lt), lt.shape[0],
+                                   ctypes.byref(cp), _native.ptr(out),
+                                   _native.stream_handle()), "dr_phong_shade")
+  return out
+
+
+def random_lights(num_lights, seed=0, r_min=4.0, r_max=5.0, center=(0, 0, 0)):
+  """Seeded lights uniform in a spherical shell, intensities U[0, 1]."""
+  rng = np.random.default_rng(seed)
+  d = rng.normal(size=(num_lights, 3))
+  d /= np.linalg.norm(d, axis=1, keepdims=True)
+  r = np.cbrt(rng.uniform(r_min ** 3, r_max ** 3, size=(num_lights, 1)))
+  pos = d * r + np.asarray(center, np.float64)
+  inten = rng.uniform(0, 1, size=(num_lights, 1))
+  return np.concatenate([pos, inten], 1).astype(np.float32)
+
+
+class MLPShader:
+  """The learned deferred shader and its on-GPU training state."""
+
+  def __init__(self, params, device):
+    lib = _native.lib()
+    n = lib.dr_mlp_num_params()
+    p = torch.as_tensor(np.asarray(params, np.float32), device=device)
+    if p.numel() != n:
+      raise ValueError(f"expected {n} parameters, got {p.numel()}")
+    self.params = p.contiguous()
+    self.packed = torch.empty(lib.dr_mlp_packed_bytes(), dtype=torch.uint8,
+                              device=device)
+    self.grad = torch.zeros_like(self.params)
+    self.m = torch.zeros_like(self.params)
+    self.v = torch.zeros_like(self.params)
+    self.loss = torch.zeros(1, dtype=torch.float32, device=device)
+    self.step_count = 0
+    self.pack()
+
+  def pack(self):
+    _native.check(_native.lib().dr_mlp_pack(_native.ptr(self.params),
+                                            _native.ptr(self.packed),
+                                            _native.stream_handle()),
+                  "dr_mlp_pack")
+
+  def forward(self, x, num_blocks=0):
+    """x [...,12] -> [...,3]."""
+    x = x.contiguous()
+    y = torch.empty(tuple(x.shape[:-1]) + (3,), dtype=torch.float32,
+                    device=x.device)
+    _native.check(
+        _native.lib().dr_mlp_forward(_native.ptr(self.packed),
+                                     _native.ptr(self.params), _native.ptr(x),
+                                     x.numel() // 12, _native.ptr(y),
+                                     num_blocks, _native.stream_handle()),
+        "dr_mlp_forward")
+    return y
+
+  def loss_and_grad(self, x, target, mask=None, scale=None, num_blocks=0):
+    """Clears and fills self.loss / self.grad for one pass over x."""
+    x = x.contiguous()
+    n = x.numel() // 12
+    t = target.contiguous()
+    mk = None if mask is None else mask.to(torch.float32).contiguous()
+    if scale is None:
+      cnt = n if mask is None else max(float(mk.sum().item()), 1.0)
+      scale = 1.0 / (3.0 * cnt)
+    self.grad.zero_()
+    self.loss.zero_()
+    _native.check(
+        _native.lib().dr_mlp_train_step(
+            _native.ptr(self.packed), _native.ptr(self.params),
+            _native.ptr(x), _native.ptr(t), _native.ptr(mk), n, float(scale),
+            _native.ptr(self.grad), _native.ptr(self.loss), num_blocks,
+            _native.stream_handle()), "dr_mlp_train_step")
+    return self.loss
+
+  def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+    self.step_count += 1
+    _native.check(
+        _native.lib().dr_mlp_adam(_native.ptr(self.params),
+                                  _native.ptr(self.grad), _native.ptr(self.m),
+                                  _native.ptr(self.v), self.step_count, lr,
+                                  beta1, beta2, eps, _native.stream_handle()),
+        "dr_mlp_adam")
+    self.pack()
+
+  def train_step(self, x, target, mask=None, scale=None, lr=1e-3):
+    loss = self.loss_and_grad(x, target, mask, scale)
+    self.adam_step(lr)
+    return loss
+
+
+def shade_with_fallback(mlp, gbuf, reference, tolerance=1e-2):
+  """Network shading with the plain shader kept where the per-pixel error
+  (max over channels) exceeds `tolerance`; returns (image, fallback mask)."""
+  y = mlp.forward(gbuf)
+  err = (y - reference).abs().amax(-1)
+  fb = err > tolerance
+  return torch.where(fb[..., None], reference, y), fb

@@ -206,6 +206,128 @@ int dr_splat_bwd(const float* xyz, const float* color, const float* mask,
                  float* d_xyz, void* workspace, size_t workspace_bytes,
                  dr_stream_t stream);
 
+
+/* ------------------------------------------------------------------------ */
+/* Fused, batched rasterize-then-splat render (forward + backward)          */
+/* ------------------------------------------------------------------------ */
+
+/* One render job: a rasterized view of a mesh with per-vertex colours and an
+ * identity shader (render.py:123-132 with shading_function returning the
+ * interpolated colours).  All pointers are device pointers; C = channels,
+ * the last channel is alpha. */
+typedef struct dr_splat_job {
+  const float* clip_vertices;   /* [V, 4] clip space, 16-byte aligned */
+  const int32_t* triangles;     /* [T, 3] */
+  const float* colors;          /* [V, C] */
+  const float* camera;          /* [4, 4] row-major, or NULL (see d_vertices) */
+  const int32_t* triangle_ids;  /* [L, H, W] from the rasterizer */
+  const float* z_buffer;        /* [L, H, W] from the rasterizer */
+  const float* barycentrics;    /* [L, H, W, 3] from the rasterizer */
+  float* image;                 /* [H, W, C] output, or NULL */
+  const float* target;          /* [H, W, C] L2 target (grad_mode 1) */
+  const float* d_image;         /* [H, W, C] upstream gradient (grad_mode 2) */
+  float* grad_scratch;          /* [H, W, 3, C + 1] per-job workspace */
+  float* d_colors;              /* [V, C] accumulated, or NULL */
+  float* d_vertices;            /* [V, 3] accumulated (M^T d clip) when camera
+                                   is set, else [V, 4] d clip; or NULL */
+  int32_t num_vertices;
+  int32_t num_triangles;
+  int32_t width;
+  int32_t height;
+  int64_t tile_begin;           /* filled by dr_splat_jobs_prepare */
+} dr_splat_job;
+
+/* Host: fills tile_begin of jobs[0..n) and returns the total tile count. */
+int dr_splat_jobs_prepare(dr_splat_job* jobs_host, int32_t num_jobs,
+                          int32_t channels, int64_t* total_tiles);
+
+/* Target pass over all jobs in one launch: interpolate colours and viewport
+ * positions (interpolate.py:24-63, splat.py:209-221), splat
+ * (splat.py:26-175) and composite (composite.py:20-37) into `image`.
+ * grad_mode 0: image only.  1: also the L2 loss sum((image - target)^2) *
+ * loss_scale accumulated into *loss (device float, caller clears) and its
+ * gradient 2 (image - target) loss_scale pushed into grad_scratch.  2: the
+ * given d_image pushed into grad_scratch.  channels 1-8, 12 or 16.
+ * No reference counterpart as one call: it fuses render.py:123-132 for an
+ * identity shader, batched where the reference runs one launch per image. */
+int dr_render_splat_forward(const dr_splat_job* jobs_device, int32_t num_jobs,
+                            int64_t total_tiles, int32_t num_layers,
+                            int32_t channels, int32_t grad_mode,
+                            float loss_scale, float* loss,
+                            dr_stream_t stream);
+
+/* Source pass: from grad_scratch (written by a forward with grad_mode 1 or
+ * 2), accumulate d_colors and d_vertices for all jobs (float atomics).  The
+ * hand-written VJP of splat.py:178-229 + interpolate.py:24-63 +
+ * transforms.py:40-78; the reference uses XLA autodiff. */
+int dr_render_splat_backward(const dr_splat_job* jobs_device,
+                             int32_t num_jobs, int64_t total_tiles,
+                             int32_t num_layers, int32_t channels,
+                             dr_stream_t stream);
+
+
+/* ------------------------------------------------------------------------ */
+/* Deferred shading: G-buffer, many-light Phong, learned MLP shader         */
+/* ------------------------------------------------------------------------ */
+
+/* G-buffer [num_pixels, 12] = world position, unit normal, unit view
+ * direction (eye - position), albedo; zeros on background.  Attributes are
+ * interpolated as dr_interpolate_fwd does; normals and view directions are
+ * normalised as transforms.l2_normalize (transforms.py:26-37) does.
+ * positions/normals/albedo [num_vertices, 3]; eye [3] (device). */
+int dr_gbuffer(const int32_t* triangle_ids, const float* barycentrics,
+               const int32_t* triangles, int32_t num_triangles,
+               const float* positions, const float* normals,
+               const float* albedo, int32_t num_vertices, const float* eye,
+               int64_t num_pixels, float* gbuffer, dr_stream_t stream);
+
+typedef struct dr_phong_params {
+  float ka, kd, ks, ns;
+  float attenuation_constant, attenuation_linear, attenuation_quadratic;
+} dr_phong_params;
+
+/* out [num_pixels, 3] = albedo * (ka + sum_i I_i decay_i (kd nol_i +
+ * ks noh_i^ns)), lights [num_lights, 4] = (x, y, z, intensity).  The
+ * many-light form of shaders.point_light (shaders.py:100-173) with the
+ * ambient term counted once. */
+int dr_phong_shade(const float* gbuffer, int64_t num_pixels,
+                   const float* lights, int32_t num_lights,
+                   const dr_phong_params* params, float* out,
+                   dr_stream_t stream);
+
+/* MLP 12 -> 64 -> 64 -> 64 -> 64 -> 3 (ReLU), bfloat16 MFMA operands, float32
+ * accumulation.  No reference counterpart (a learned replacement for the
+ * closed-form shader above).  Parameters are a flat float32 vector of
+ * dr_mlp_num_params() entries: W1[12,64] b1 W2[64,64] b2 W3 b3 W4 b4
+ * W5[64,3] b5, W[k_in][unit_out]. */
+int32_t dr_mlp_num_params(void);
+
+/* Bytes of the packed bfloat16 operand image written by dr_mlp_pack. */
+size_t dr_mlp_packed_bytes(void);
+
+/* Packs float32 parameters into the MFMA operand image (forward and
+ * backward orders); call after every parameter update. */
+int dr_mlp_pack(const float* params, void* packed, dr_stream_t stream);
+
+/* y [num_pixels, 3] = MLP(x [num_pixels, 12]).  num_blocks <= 0: default. */
+int dr_mlp_forward(const void* packed, const float* params, const float* x,
+                   int64_t num_pixels, float* y, int32_t num_blocks,
+                   dr_stream_t stream);
+
+/* One training pass: loss += loss_scale * sum_p mask[p] |y_p - target_p|^2
+ * and grad += d loss / d params (float32, accumulated: the caller clears
+ * grad and loss).  mask [num_pixels] may be NULL (all ones). */
+int dr_mlp_train_step(const void* packed, const float* params, const float* x,
+                      const float* target, const float* mask,
+                      int64_t num_pixels, float loss_scale, float* grad,
+                      float* loss, int32_t num_blocks, dr_stream_t stream);
+
+/* Adam update of params with moments m, v (all float32, num_params);
+ * step counts from 1. */
+int dr_mlp_adam(float* params, const float* grad, float* m, float* v,
+                int32_t step, float lr, float beta1, float beta2, float eps,
+                dr_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

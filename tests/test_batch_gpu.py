@@ -1,0 +1,120 @@
+"""Batched fused render (diffren_amd.batch) vs the unfused per-view path and
+the float64 oracle; HIP-graph replays vs eager."""
+import numpy as np
+import pytest
+import torch
+
+import helpers
+from diffren_amd import batch
+from diffren_amd import render
+from diffren_amd import scenes
+from oracle import raster_oracle
+from oracle import splat_oracle as so
+
+pytestmark = pytest.mark.gpu
+
+
+def _meshes(device):
+  out = []
+  for seed, sub in ((0, 3), (5, 2)):
+    s = scenes.icosphere_scene(seed=seed, subdivisions=sub)
+    out.append(dict(
+        vertices=torch.as_tensor(s["vertices"], device=device),
+        triangles=torch.as_tensor(s["triangles"], device=device),
+        colors=torch.as_tensor(s["colors"], device=device),
+        camera=torch.as_tensor(s["camera"], device=device), scene=s))
+  return out
+
+
+def _jobs(meshes):
+  rot = torch.tensor(scenes.random_rotation(np.random.default_rng(9)),
+                     dtype=torch.float32, device=meshes[0]["camera"].device)
+  view = torch.eye(4, device=rot.device)
+  view[:3, :3] = rot
+  # Views of a mesh centred at z=-3: rotate about the mesh centre.
+  shift = torch.eye(4, device=rot.device)
+  shift[2, 3] = -3.0
+  unshift = torch.eye(4, device=rot.device)
+  unshift[2, 3] = 3.0
+  cam2 = meshes[0]["camera"] @ shift @ view @ unshift
+  return [(0, meshes[0]["camera"], 96, 64), (0, cam2.contiguous(), 128, 128),
+          (1, meshes[1]["camera"], 40, 72)]
+
+
+def _unfused(meshes, jobs, L, targets):
+  """Per-view render_triangles + autograd (the plain path)."""
+  vs = [m["vertices"].clone().requires_grad_(True) for m in meshes]
+  cs = [m["colors"].clone().requires_grad_(True) for m in meshes]
+  total = 0.0
+  imgs = []
+  npix = sum(w * h * meshes[mi]["colors"].shape[1] for mi, _, w, h in jobs)
+  for (mi, cam, w, h), tgt in zip(jobs, targets):
+    img = render.render_triangles(vs[mi], {"c": cs[mi]},
+                                  meshes[mi]["triangles"], cam, w, h,
+                                  lambda a: a["c"], num_layers=L)
+    imgs.append(img)
+    total = total + ((img - tgt) ** 2).sum() / npix
+  total.backward()
+  return imgs, total, [v.grad for v in vs], [c.grad for c in cs]
+
+
+@pytest.mark.parametrize("L", [1, 3])
+def test_fused_matches_unfused_and_oracle(L, device):
+  meshes = _meshes(device)
+  jobs = _jobs(meshes)
+  r = batch.BatchedSplatRenderer(meshes, jobs, L)
+  gen = torch.Generator(device="cpu").manual_seed(L)
+  for t in r.targets:
+    t.copy_(torch.rand(t.shape, generator=gen))
+  r.step()
+  torch.cuda.synchronize()
+  imgs, loss, dv, dc = _unfused(meshes, jobs, L, r.targets)
+  for a, b in zip(r.images, imgs):
+    assert helpers.rel_err(a.cpu(), b.detach().cpu()) <= 1e-5
+  assert abs(r.loss.item() - loss.item()) <= 1e-5 * abs(loss.item())
+  for a, b in zip(r.d_vertices + r.d_colors, dv + dc):
+    assert helpers.rel_err(a.cpu(), b.cpu()) <= 1e-4
+  # Float64 oracle of the same loss, rasterization from the oracle.
+  v64 = [torch.as_tensor(m["scene"]["vertices"],
+                         dtype=torch.float64).requires_grad_(True)
+         for m in meshes]
+  c64 = [torch.as_tensor(m["scene"]["colors"],
+                         dtype=torch.float64).requires_grad_(True)
+         for m in meshes]
+  total = 0.0
+  for i, ((mi, cam, w, h), tgt) in enumerate(zip(jobs, r.targets)):
+    s = meshes[mi]["scene"]
+    cam_np = cam.cpu().numpy()
+    clip32 = so.transform(cam_np, s["vertices"], torch.float32).numpy()
+    ids, _, b = raster_oracle.rasterize(clip32, s["triangles"], w, h, L, 1)
+    # The rasterizer is bit-exact, so the GPU's frame is the oracle's.
+    assert np.array_equal(ids, r.ids[i].cpu().numpy())
+    o = so.render_splat(v64[mi], c64[mi], cam_np, ids, b, s["triangles"])
+    total = total + ((o - tgt.cpu().double()) ** 2).sum() * r.loss_scale
+  total.backward()
+  assert abs(r.loss.item() - total.item()) <= 1e-5 * abs(total.item())
+  for a, b in zip(r.d_vertices + r.d_colors,
+                  [v.grad for v in v64] + [c.grad for c in c64]):
+    assert helpers.rel_err(a.cpu(), b) <= 1e-4
+
+
+def test_graph_replays_match_eager(device):
+  meshes = _meshes(device)
+  jobs = _jobs(meshes)
+  r = batch.BatchedSplatRenderer(meshes, jobs, 3)
+  for t in r.targets:
+    t.uniform_()
+  r.step()
+  torch.cuda.synchronize()
+  want = [t.clone() for t in r.d_vertices + r.d_colors + r.images]
+  want_loss = r.loss.item()
+  r.capture()
+  for k in range(4):
+    r.replay()
+    torch.cuda.synchronize()
+    got = r.d_vertices + r.d_colors + r.images
+    for a, b in zip(got, want):
+      assert helpers.rel_err(a.cpu(), b.cpu()) <= 1e-5, k
+    assert abs(r.loss.item() - want_loss) <= 1e-5 * abs(want_loss)
+    # Eager work between replays.
+    torch.ones(1000, device=device).sum().item()

@@ -1,0 +1,112 @@
+"""G-buffer, many-light Phong and the learned MLP shader vs their oracles."""
+import numpy as np
+import pytest
+import torch
+
+import helpers
+from diffren_amd import rasterize
+from diffren_amd import scenes
+from diffren_amd import shading
+from oracle import mlp_oracle as mo
+from oracle import splat_oracle as so
+
+pytestmark = pytest.mark.gpu
+
+
+def _gbuffer_scene(device, seed=0, res=128):
+  s = scenes.icosphere_scene(seed=seed)
+  v = torch.as_tensor(s["vertices"], device=device)
+  tris = torch.as_tensor(s["triangles"], device=device)
+  cam = torch.as_tensor(s["camera"], device=device)
+  fb = rasterize.rasterize_triangles(v, tris, cam, res, res, 1)
+  nrm = scenes.vertex_normals(s["vertices"].astype(np.float64),
+                              s["triangles"]).astype(np.float32)
+  alb = s["colors"][:, :3].copy()
+  g = shading.gbuffer(fb, v, torch.as_tensor(nrm, device=device),
+                      torch.as_tensor(alb, device=device), (0.0, 0.0, 0.0))
+  return s, fb, nrm, alb, g
+
+
+def test_gbuffer_matches_interpolation(device):
+  s, fb, nrm, alb, g = _gbuffer_scene(device)
+  ids = fb.triangle_id[..., 0].cpu().numpy()
+  b = fb.barycentrics.cpu().numpy()
+  pos = so.interpolate(torch.as_tensor(s["vertices"]), ids, b,
+                       s["triangles"]).numpy()
+  n = so.interpolate(torch.as_tensor(nrm), ids, b, s["triangles"]).numpy()
+  a = so.interpolate(torch.as_tensor(alb), ids, b, s["triangles"]).numpy()
+  fg = ids >= 0
+  n = n / np.sqrt((n * n).sum(-1, keepdims=True) + 1e-12)
+  vd = -pos / np.sqrt((pos * pos).sum(-1, keepdims=True) + 1e-12)
+  want = np.concatenate([pos, n, vd, a], -1) * fg[..., None]
+  assert helpers.rel_err(g.cpu().numpy(), want) <= 1e-5
+
+
+def test_phong_matches_oracle(device):
+  _, _, _, _, g = _gbuffer_scene(device)
+  lights = shading.random_lights(64, seed=1)
+  out = shading.phong(g, torch.as_tensor(lights, device=device))
+  want = mo.phong(g.cpu().numpy().reshape(-1, 12), lights).reshape(
+      out.shape)
+  assert helpers.rel_err(out.cpu().numpy(), want) <= 1e-5
+
+
+def test_mlp_forward_matches_oracle(device):
+  rng = np.random.default_rng(0)
+  p = mo.init_params(3)
+  # Odd pixel count: a partial last chunk.
+  x = rng.normal(size=(1000 * 32 + 17, 12)).astype(np.float32)
+  net = shading.MLPShader(p, device)
+  y = net.forward(torch.as_tensor(x, device=device)).cpu().numpy()
+  want = mo.forward(p, x)
+  assert helpers.rel_err(y, want) <= 2e-3
+
+
+def test_mlp_gradient_matches_oracle(device):
+  rng = np.random.default_rng(1)
+  p = mo.init_params(4)
+  n = 4099
+  x = rng.normal(size=(n, 12)).astype(np.float32)
+  t = rng.normal(size=(n, 3)).astype(np.float32)
+  mask = (rng.uniform(size=n) > 0.3).astype(np.float32)
+  scale = 1.0 / (3 * n)
+  net = shading.MLPShader(p, device)
+  net.loss_and_grad(torch.as_tensor(x, device=device),
+                    torch.as_tensor(t, device=device),
+                    torch.as_tensor(mask, device=device), scale)
+  loss, g = mo.loss_and_grad(p, x, t, mask, scale)
+  got = net.grad.cpu().numpy()
+  assert abs(net.loss.item() - loss) <= 1e-3 * abs(loss)
+  assert np.linalg.norm(got - g) <= 1e-2 * np.linalg.norm(g)
+  # Per-layer weight gradients individually.
+  for (a, b), _, _ in mo.param_slices()[0]:
+    assert np.linalg.norm(got[a:b] - g[a:b]) <= 2e-2 * np.linalg.norm(g[a:b])
+
+
+def test_mlp_adam_matches_oracle(device):
+  p = mo.init_params(5)
+  rng = np.random.default_rng(2)
+  g = rng.normal(size=p.size).astype(np.float32)
+  net = shading.MLPShader(p, device)
+  net.grad.copy_(torch.as_tensor(g))
+  net.adam_step(lr=1e-3)
+  want, _, _ = mo.adam(p, g, np.zeros_like(p), np.zeros_like(p), 1, lr=1e-3)
+  np.testing.assert_allclose(net.params.cpu().numpy(), want, rtol=1e-5,
+                             atol=1e-7)
+
+
+def test_mlp_training_reduces_error(device):
+  _, fb, _, _, g = _gbuffer_scene(device, seed=2, res=128)
+  lights = torch.as_tensor(shading.random_lights(64, seed=1), device=device)
+  ref = shading.phong(g, lights)
+  mask = (fb.triangle_id[..., 0] >= 0).float().reshape(-1)
+  x = g.reshape(-1, 12)
+  t = ref.reshape(-1, 3)
+  net = shading.MLPShader(mo.init_params(0), device)
+  first = net.train_step(x, t, mask, lr=3e-3).item()
+  for _ in range(300):
+    last = net.train_step(x, t, mask, lr=3e-3).item()
+  assert last < 0.2 * first
+  img, fb_mask = shading.shade_with_fallback(net, g, ref, 1e-2)
+  err = (img - ref).abs().amax(-1)
+  assert err.max().item() <= 1e-2
