@@ -248,10 +248,12 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
     const float* __restrict__ target, const float* __restrict__ mask,
     int64_t N, float scale, float* __restrict__ grad,
     float* __restrict__ loss) {
-  extern __shared__ float lds[];
+  extern __shared__ __attribute__((aligned(16))) float lds[];
   float* g = lds;                       // [kParams] gradient accumulators
   float* sb = g + kParams + 1;          // [5 * 64] biases
-  __bf16* act_all = reinterpret_cast<__bf16*>(sb + 5 * 64 + 3);
+  // kParams + 1 + 5 * 64 = 13828 floats: the bf16 arrays start 16-byte
+  // aligned, as their 16-byte operand reads need.
+  __bf16* act_all = reinterpret_cast<__bf16*>(sb + 5 * 64);
   const int wave = threadIdx.x >> 6;
   __bf16* act = act_all + wave * (kActWave + kDltWave);
   __bf16* dlt = act + kActWave;
@@ -403,8 +405,11 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
   p[i] = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+static_assert((kParams + 1 + 5 * 64) % 4 == 0, "LDS bf16 stage alignment");
+static_assert(((kActWave + kDltWave) * 2) % 16 == 0, "per-wave alignment");
+
 size_t train_lds_bytes() {
-  return sizeof(float) * (kParams + 1 + 5 * 64 + 3) +
+  return sizeof(float) * (kParams + 1 + 5 * 64) +
          sizeof(__bf16) * kWaves * (kActWave + kDltWave);
 }
 

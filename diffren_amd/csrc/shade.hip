@@ -145,8 +145,8 @@ int dr_phong_shade(const float* gbuffer, int64_t num_pixels,
                    const dr_phong_params* params, float* out,
                    dr_stream_t stream) {
   DR_CHECK_ARG(num_pixels >= 0, "negative size");
-  DR_CHECK_ARG(num_lights >= 0 && num_lights <= 4096,
-               "num_lights must be in [0, 4096]");
+  DR_CHECK_ARG(num_lights >= 0 && num_lights <= 2048,
+               "num_lights must be in [0, 2048]");
   DR_CHECK_ARG(params != nullptr, "params is null");
   if (num_pixels == 0) return DR_OK;
   DR_CHECK_ARG(gbuffer && out && (num_lights == 0 || lights), "null pointer");
