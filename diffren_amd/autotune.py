@@ -1,0 +1,103 @@
+"""Autotuner for the tile rasterizer: screen-tile shape and bin capacity.
+
+For a batch of raster jobs it times every candidate tile shape with HIP
+events on the current stream (after one warm-up launch), keeps the fastest,
+then sizes the bin capacity from the entry count that configuration actually
+needed (dr_raster_bin_stats) plus headroom, so the default over-allocation
+(3 entries per triangle) shrinks to what the workload uses.  Results are cached
+in memory per workload signature (image sizes, triangle counts, layers,
+culling) and optionally in a JSON file.
+"""
+import ctypes
+import json
+import os
+
+import torch
+
+from diffren_amd import _native
+
+CANDIDATE_TILES = [(8, 8), (16, 16), (32, 8), (32, 16), (64, 8), (32, 32)]
+HEADROOM = 1.25
+
+_cache = {}
+
+
+def signature(jobs, num_layers, culling):
+  """jobs: list of (width, height, num_triangles)."""
+  return json.dumps([sorted(map(list, jobs)), int(num_layers), int(culling)])
+
+
+def _load(path):
+  if path and os.path.exists(path):
+    with open(path) as f:
+      _cache.update(json.load(f))
+
+
+def _save(path):
+  if path:
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
+      json.dump(_cache, f)
+
+
+def time_launch(fn, reps):
+  start = torch.cuda.Event(enable_timing=True)
+  end = torch.cuda.Event(enable_timing=True)
+  fn()
+  start.record()
+  for _ in range(reps):
+    fn()
+  end.record()
+  torch.cuda.synchronize()
+  return start.elapsed_time(end) / reps
+
+
+def tune_raster(table_host, num_jobs, num_layers, culling, device,
+                job_sizes, reps=5, cache_path=None, candidates=None):
+  """Returns (tile_w, tile_h, bin_capacity, timings_ms).
+
+  table_host: ctypes array of dr_raster_job with the device pointers filled;
+  job_sizes: list of (width, height, num_triangles) for the signature.
+  """
+  _load(cache_path)
+  key = signature(job_sizes, num_layers, culling)
+  if key in _cache:
+    e = _cache[key]
+    return e["tile"][0], e["tile"][1], e["capacity"], e["timings"]
+  lib = _native.lib()
+  timings = {}
+  best = None
+  for tw, th in (candidates or CANDIDATE_TILES):
+    if num_layers * tw * th * 8 > 65536:
+      continue
+    cfg = _native.RasterConfig(tw, th, 0)
+    tt, tiles = ctypes.c_int64(), ctypes.c_int64()
+    _native.check(lib.dr_raster_jobs_prepare(table_host, num_jobs,
+                                             ctypes.byref(cfg),
+                                             ctypes.byref(tt),
+                                             ctypes.byref(tiles)),
+                  "dr_raster_jobs_prepare")
+    table = _native.table_to_device(table_host, device)
+    ws = _native.workspace(lib.dr_raster_workspace_size(
+        tt.value, tiles.value, ctypes.byref(cfg)), device)
+
+    def launch():
+      _native.check(lib.dr_rasterize_triangles_batched(
+          _native.ptr(table), num_jobs, tt.value, tiles.value, num_layers,
+          culling, ctypes.byref(cfg), _native.ptr(ws), ws.numel(),
+          _native.stream_handle()), "dr_rasterize_triangles_batched")
+
+    ms = time_launch(launch, reps)
+    need, over = ctypes.c_int64(), ctypes.c_int32()
+    _native.check(lib.dr_raster_bin_stats(_native.ptr(ws), ctypes.byref(need),
+                                          ctypes.byref(over),
+                                          _native.stream_handle()),
+                  "dr_raster_bin_stats")
+    timings[f"{tw}x{th}"] = ms
+    if best is None or ms < best[0]:
+      best = (ms, tw, th, need.value)
+  _, tw, th, need = best
+  capacity = int(need * HEADROOM) + 4096
+  _cache[key] = dict(tile=[tw, th], capacity=capacity, timings=timings)
+  _save(cache_path)
+  return tw, th, capacity, timings

@@ -33,7 +33,7 @@ class BatchedSplatRenderer:
   """
 
   def __init__(self, meshes, jobs, num_layers, tile=None, bin_capacity=0,
-               loss_scale=None):
+               loss_scale=None, autotune_cache=None):
     self.meshes = meshes
     self.jobs = jobs
     self.L = int(num_layers)
@@ -68,6 +68,8 @@ class BatchedSplatRenderer:
       npix += w * h * C
     self.loss_scale = 1.0 / npix if loss_scale is None else float(loss_scale)
     self.cameras = [j[1].to(torch.float32).contiguous() for j in jobs]
+    self.autotune_cache = autotune_cache
+    self.autotune_timings = None
     self._plan_raster(tile, bin_capacity)
     self._plan_splat()
     self.graph = None
@@ -76,7 +78,24 @@ class BatchedSplatRenderer:
 
   def _plan_raster(self, tile, bin_capacity):
     lib = _native.lib()
+    if tile == "auto":
+      tile, bin_capacity = self._autotune()
     self.raster_config = rasterize._config(tile, bin_capacity)
+    n = len(self.jobs)
+    table = self._raster_table()
+    tt, tiles = ctypes.c_int64(), ctypes.c_int64()
+    _native.check(lib.dr_raster_jobs_prepare(table, n,
+                                             ctypes.byref(self.raster_config),
+                                             ctypes.byref(tt),
+                                             ctypes.byref(tiles)),
+                  "dr_raster_jobs_prepare")
+    self.raster_table = _native.table_to_device(table, self.device)
+    self.total_triangles, self.raster_tiles = tt.value, tiles.value
+    nbytes = lib.dr_raster_workspace_size(tt.value, tiles.value,
+                                          ctypes.byref(self.raster_config))
+    self.raster_ws = _native.workspace(nbytes, self.device)
+
+  def _raster_table(self):
     n = len(self.jobs)
     table = (_native.RasterJob * n)()
     for i, (mi, _, w, h) in enumerate(self.jobs):
@@ -90,17 +109,19 @@ class BatchedSplatRenderer:
       table[i].num_triangles = m["triangles"].shape[0]
       table[i].width = w
       table[i].height = h
-    tt, tiles = ctypes.c_int64(), ctypes.c_int64()
-    _native.check(lib.dr_raster_jobs_prepare(table, n,
-                                             ctypes.byref(self.raster_config),
-                                             ctypes.byref(tt),
-                                             ctypes.byref(tiles)),
-                  "dr_raster_jobs_prepare")
-    self.raster_table = _native.table_to_device(table, self.device)
-    self.total_triangles, self.raster_tiles = tt.value, tiles.value
-    nbytes = lib.dr_raster_workspace_size(tt.value, tiles.value,
-                                          ctypes.byref(self.raster_config))
-    self.raster_ws = _native.workspace(nbytes, self.device)
+    return table
+
+  def _autotune(self):
+    """Tile shape and bin capacity from diffren_amd.autotune."""
+    from diffren_amd import autotune
+    self.transform()
+    sizes = [(w, h, self.meshes[mi]["triangles"].shape[0])
+             for mi, _, w, h in self.jobs]
+    tw, th, cap, timings = autotune.tune_raster(
+        self._raster_table(), len(self.jobs), self.L, 1, self.device, sizes,
+        cache_path=self.autotune_cache)
+    self.autotune_timings = timings
+    return (tw, th), cap
 
   def _plan_splat(self):
     lib = _native.lib()

@@ -245,20 +245,17 @@ __device__ __forceinline__ uint32_t z_order(float z) {
 // ---------------------------------------------------------------------------
 // Jobs.
 
+// All kernels read jobs from a table in global memory (the single-job entry
+// point stores its one job into the workspace first), so a job is never
+// selected between a kernel-argument copy and a table entry.
 struct Jobs {
-  const dr_raster_job* table;  // device table, or null for a single job
-  dr_raster_job one;
+  const dr_raster_job* table;  // device table of `count` jobs
   int32_t count;
 };
-
-__device__ __forceinline__ dr_raster_job load_job(const Jobs& J, int j) {
-  return J.table ? J.table[j] : J.one;
-}
 
 // Largest j with begin(j) <= flat.
 template <bool kTiles>
 __device__ __forceinline__ int find_job(const Jobs& J, int64_t flat) {
-  if (!J.table) return 0;
   int lo = 0, hi = J.count - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -267,6 +264,11 @@ __device__ __forceinline__ int find_job(const Jobs& J, int64_t flat) {
     if (b <= flat) lo = mid; else hi = mid - 1;
   }
   return lo;
+}
+
+__global__ void store_job_kernel(dr_raster_job job,
+                                 dr_raster_job* __restrict__ slot) {
+  *slot = job;
 }
 
 // Loads triangle `t` of `job`; false if an index is out of range.
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void bin_kernel(
   int64_t tile_base = 0;
   if (flat < total_tris) {
     const int j = find_job<false>(J, flat);
-    const dr_raster_job job = load_job(J, j);
+    const dr_raster_job& job = J.table[j];
     t = int(flat - job.triangle_begin);
     float4 v[3];
     Box b;
@@ -444,16 +446,92 @@ __global__ __launch_bounds__(kBlock) void scan_add_kernel(
 // Inserts key into the L-deep sorted list of LDS slots at `p` (slot l at
 // keys[l * TP + p]).  Each slot keeps the minimum it has seen and passes the
 // larger key down, so after all insertions slot l holds the (l+1)-th smallest
-// key regardless of arrival order.
-__device__ __forceinline__ void insert_key(uint64_t* keys, int tp, int p,
-                                          int L, uint64_t key) {
+// (z, id) key regardless of arrival order.
+//
+// That equals the reference's id-ordered insertion sort with strict '<'
+// (rasterize_triangles_impl.cc:95-114) except at exact z ties: a displaced
+// entry cascading into a run of equal-z entries moves past them, so which of
+// the tied fragments survive, and in what order, depends on the insertion
+// history.  Any comparison of two keys with equal z therefore flags the pixel
+// for an exact sequential replay (replay_pixel).  A tie that can change the
+// result always produces such a comparison: if keys A < B have equal z and
+// adjacent ranks r, r+1 <= L, both reach slot r, and whichever arrives second
+// meets the other there (slot r only ever holds keys >= A, and after B has
+// arrived, keys <= B).
+__device__ __forceinline__ void insert_key(uint64_t* keys, uint32_t* ties,
+                                           int tp, int p, int L,
+                                           uint64_t key) {
   for (int l = 0; l < L; ++l) {
     const uint64_t old = atomicMin(
         reinterpret_cast<unsigned long long*>(&keys[l * tp + p]),
         static_cast<unsigned long long>(key));
     if (old == kEmpty) return;
+    if ((old >> 32) == (key >> 32)) atomicOr(&ties[p >> 5], 1u << (p & 31));
     key = old > key ? old : key;
   }
+}
+
+// Re-resolves one pixel by replaying the reference's insertion sort
+// (rasterize_triangles_impl.cc:95-114) over its fragments in ascending
+// triangle id.  Only fragments with z at most the largest kept z take part
+// when the list is full: a fragment with larger z is never kept, and until it
+// is pushed out it acts on the smaller ones exactly like an empty slot (both
+// compare greater), so it cannot change which of them survive or their order.
+// Rare (pixels with exact z ties), so it rescans the tile's bin once per
+// candidate to visit candidates in id order without sorting.
+__device__ void replay_pixel(const dr_raster_job& job, int ix, int iy,
+                             float sx2, float sy2, int cull, bool overflow,
+                             uint32_t cnt, uint64_t off,
+                             const uint32_t* __restrict__ bins, int L,
+                             uint64_t* slot, int tp) {
+  const uint64_t last = slot[(L - 1) * tp];
+  const uint32_t zmax = last == kEmpty ? 0xffffffffu : uint32_t(last >> 32);
+  const int64_t items = overflow ? int64_t(job.num_triangles) : int64_t(cnt);
+  float lz[kMaxLayers];
+  int lid[kMaxLayers];
+  for (int l = 0; l < L; ++l) {
+    lz[l] = 1.0f;
+    lid[l] = -1;
+  }
+  int prev = -1;
+  for (;;) {
+    int next = INT32_MAX;
+    float next_z = 1.0f;
+    for (int64_t i = 0; i < items; ++i) {
+      const int t = overflow ? int(i) : int(bins[off + i]);
+      if (t <= prev || t >= next) continue;
+      float4 v[3];
+      Box b;
+      if (!load_triangle(job, t, v) ||
+          !triangle_bbox(v, job.width, job.height, b))
+        continue;
+      if (ix < b.left || ix >= b.right || iy < b.bottom || iy >= b.top)
+        continue;
+      int64_t m[9], r[9];
+      fixed_matrix(v, m);
+      adjugate(m, cull, r);
+      Fragment f;
+      if (!shade_pixel(ix, iy, sx2, sy2, r, v, f)) continue;
+      if (z_order(f.z) > zmax) continue;
+      next = t;
+      next_z = f.z;
+    }
+    if (next == INT32_MAX) break;
+    float zn = next_z;
+    int idn = next;
+    for (int l = 0; l < L; ++l) {
+      if (zn < lz[l]) {
+        const float tz = lz[l]; lz[l] = zn; zn = tz;
+        const int ti = lid[l]; lid[l] = idn; idn = ti;
+      }
+      if (zn == 1.0f) break;
+    }
+    prev = next;
+  }
+  for (int l = 0; l < L; ++l)
+    slot[l * tp] = lid[l] < 0 ? kEmpty
+                              : (uint64_t(z_order(lz[l])) << 32) |
+                                    uint32_t(lid[l]);
 }
 
 struct TriRect {
@@ -471,10 +549,11 @@ __global__ __launch_bounds__(kBlock) void raster_tile_kernel(
   extern __shared__ uint64_t keys[];  // [L][TP]
   __shared__ TriRect queue[kBlock];
   __shared__ int queue_n;
+  __shared__ uint32_t ties[(TP + 31) / 32];
 
   const int64_t flat_tile = blockIdx.x;
   const int j = find_job<true>(J, flat_tile);
-  const dr_raster_job job = load_job(J, j);
+  const dr_raster_job& job = J.table[j];
   const int W = job.width, H = job.height;
   const int ntx = tiles_x(job, TW);
   const int local = int(flat_tile - job.tile_begin);
@@ -484,6 +563,7 @@ __global__ __launch_bounds__(kBlock) void raster_tile_kernel(
   const float sy2 = 2.0f / static_cast<float>(H);
 
   for (int i = threadIdx.x; i < L * TP; i += kBlock) keys[i] = kEmpty;
+  for (int i = threadIdx.x; i < (TP + 31) / 32; i += kBlock) ties[i] = 0u;
   const uint32_t cnt = counts[flat_tile];
   const uint64_t off = offsets[flat_tile];
   const bool overflow = off + cnt > capacity;
@@ -516,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void raster_tile_kernel(
             for (int ix = rect.x0; ix < rect.x1; ++ix) {
               Fragment f;
               if (shade_pixel(ix, iy, sx2, sy2, r, v, f))
-                insert_key(keys, TP, (iy - y0) * TW + (ix - x0), L,
+                insert_key(keys, ties, TP, (iy - y0) * TW + (ix - x0), L,
                            (uint64_t(z_order(f.z)) << 32) | uint32_t(t));
             }
         }
@@ -547,11 +627,21 @@ __global__ __launch_bounds__(kBlock) void raster_tile_kernel(
         const int ix = rq.x0 + k % rw, iy = rq.y0 + k / rw;
         Fragment f;
         if (shade_pixel(ix, iy, sx2, sy2, r, v, f))
-          insert_key(keys, TP, (iy - y0) * TW + (ix - x0), L,
+          insert_key(keys, ties, TP, (iy - y0) * TW + (ix - x0), L,
                      (uint64_t(z_order(f.z)) << 32) | uint32_t(rq.tri));
       }
     }
     __syncthreads();
+  }
+  __syncthreads();
+
+  // Exact replay of the reference's insertion order at pixels with z ties.
+  for (int p = threadIdx.x; p < TP; p += kBlock) {
+    if (!(ties[p >> 5] & (1u << (p & 31)))) continue;
+    const int ix = x0 + p % TW, iy = y0 + p / TW;
+    if (ix >= x1 || iy >= y1) continue;
+    replay_pixel(job, ix, iy, sx2, sy2, cull, overflow, cnt, off, bins, L,
+                 keys + p, TP);
   }
   __syncthreads();
 
@@ -595,6 +685,7 @@ struct Workspace {
   uint32_t* block_sums;
   uint32_t* status;
   uint32_t* bins;
+  dr_raster_job* job;  // the single-job entry point's one-entry table
   size_t bytes;
 };
 
@@ -624,6 +715,8 @@ Workspace carve(void* base, int64_t total_tiles, uint64_t capacity) {
   o += align256(sizeof(uint32_t) * scan_blocks(total_tiles));
   w.bins = reinterpret_cast<uint32_t*>(p + o);
   o += align256(sizeof(uint32_t) * capacity);
+  w.job = reinterpret_cast<dr_raster_job*>(p + o);
+  o += align256(sizeof(dr_raster_job));
   w.bytes = o;
   return w;
 }
@@ -672,7 +765,10 @@ int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
   return kOk;
 }
 
-int dispatch(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
+// `table` is a device table of `count` jobs, or null with `single` the one
+// job of dr_rasterize_triangles (stored into the workspace here).
+int dispatch(const dr_raster_job* table, const dr_raster_job* single,
+             int count, int64_t total_tris, int64_t total_tiles, int L,
              int cull, const dr_raster_config& c, void* ws, size_t ws_bytes,
              hipStream_t s) {
   DR_CHECK_ARG(L >= 1 && L <= kMaxLayers, "num_layers must be in [1, 8]");
@@ -694,6 +790,12 @@ int dispatch(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
     return fail(kWorkspaceTooSmall,
                 "workspace too small: need " + std::to_string(w.bytes) +
                     " bytes, got " + std::to_string(ws_bytes));
+  if (table == nullptr) {
+    store_job_kernel<<<1, 1, 0, s>>>(*single, w.job);
+    DR_CHECK_LAUNCH("raster store job");
+    table = w.job;
+  }
+  const Jobs J{table, count};
 #define DR_TILE_CASE(A, B)                                              \
   if (c.tile_w == A && c.tile_h == B)                                   \
     return launch_all<A, B>(J, total_tris, total_tiles, L, cull, w,     \
@@ -778,9 +880,8 @@ int dr_rasterize_triangles(const float* vertices, int32_t num_vertices,
   int64_t tt = 0, tiles = 0;
   const int rc = dr_raster_jobs_prepare(&job, 1, &c, &tt, &tiles);
   if (rc != DR_OK) return rc;
-  dr::Jobs J{nullptr, job, 1};
-  return dr::dispatch(J, tt, tiles, num_layers, face_culling_mode, c,
-                      workspace, workspace_bytes,
+  return dr::dispatch(nullptr, &job, 1, tt, tiles, num_layers,
+                      face_culling_mode, c, workspace, workspace_bytes,
                       static_cast<hipStream_t>(stream));
 }
 
@@ -793,9 +894,9 @@ int dr_rasterize_triangles_batched(const dr_raster_job* jobs_device,
                                    dr_stream_t stream) {
   const dr_raster_config c = dr::resolve_config(config);
   DR_CHECK_ARG(jobs_device != nullptr && num_jobs > 0, "need jobs");
-  dr::Jobs J{jobs_device, dr_raster_job{}, num_jobs};
-  return dr::dispatch(J, total_triangles, total_tiles, num_layers,
-                      face_culling_mode, c, workspace, workspace_bytes,
+  return dr::dispatch(jobs_device, nullptr, num_jobs, total_triangles,
+                      total_tiles, num_layers, face_culling_mode, c,
+                      workspace, workspace_bytes,
                       static_cast<hipStream_t>(stream));
 }
 

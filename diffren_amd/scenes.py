@@ -221,3 +221,27 @@ def sphere_features(vertices, triangles, channels=8, seed=0):
   n = vertex_normals(vertices.astype(np.float64), triangles)
   f = rng.normal(size=(vertices.shape[0], channels))
   return n.astype(np.float32), f.astype(np.float32)
+
+
+def batching_workload(num_meshes=64, t_min=1_000, t_max=500_000, seed=0,
+                      resolutions=(128, 256, 512, 1024)):
+  """Independent noise-displaced spheres with log-uniform triangle counts,
+  each with its own random orbit camera and square resolution.
+
+  Returns a list of dicts: vertices [V,3], triangles [T,3], colors [V,4]
+  (RGB uniform, alpha 1), camera [4,4], resolution.  Triangle counts are
+  2 * rows * cols with cols = 2 * rows, the UV sphere closest to the draw.
+  """
+  rng = np.random.default_rng(seed)
+  out = []
+  for i in range(num_meshes):
+    t = math.exp(rng.uniform(math.log(t_min), math.log(t_max)))
+    rows = max(2, int(round(math.sqrt(t / 4.0))))
+    v, tris = uv_sphere_mesh(rows, 2 * rows, seed=seed * 1000 + i)
+    col = np.concatenate([rng.uniform(0, 1, (v.shape[0], 3)),
+                          np.ones((v.shape[0], 1))], 1).astype(np.float32)
+    cam = orbit_cameras(1, seed=seed * 1000 + i)[0]
+    res = int(rng.choice(resolutions))
+    out.append(dict(vertices=v, triangles=tris, colors=col, camera=cam,
+                    resolution=res))
+  return out

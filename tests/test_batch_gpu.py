@@ -118,3 +118,32 @@ def test_graph_replays_match_eager(device):
     assert abs(r.loss.item() - want_loss) <= 1e-5 * abs(want_loss)
     # Eager work between replays.
     torch.ones(1000, device=device).sum().item()
+
+
+def test_uneven_batch_autotuned_matches_per_view(device):
+  """The batching configuration at reduced scale: meshes of uneven triangle
+  counts and resolutions in one batch, autotuned tiles, vs per-view renders
+  of the plain path."""
+  from diffren_amd import scenes as sc
+  work = sc.batching_workload(num_meshes=6, t_min=500, t_max=20_000, seed=3,
+                              resolutions=(64, 96, 128, 200))
+  meshes = [dict(vertices=torch.as_tensor(m["vertices"], device=device),
+                 triangles=torch.as_tensor(m["triangles"], device=device),
+                 colors=torch.as_tensor(m["colors"], device=device))
+            for m in work]
+  jobs = [(i, torch.as_tensor(m["camera"], device=device), m["resolution"],
+           m["resolution"]) for i, m in enumerate(work)]
+  r = batch.BatchedSplatRenderer(meshes, jobs, 3, tile="auto")
+  assert r.autotune_timings
+  for t in r.targets:
+    t.uniform_()
+  r.step()
+  torch.cuda.synchronize()
+  imgs, loss, dv, dc = _unfused(meshes, jobs, 3, r.targets)
+  for a, b in zip(r.images, imgs):
+    assert helpers.rel_err(a.cpu(), b.detach().cpu()) <= 1e-5
+  assert abs(r.loss.item() - loss.item()) <= 1e-5 * abs(loss.item())
+  for a, b in zip(r.d_vertices + r.d_colors, dv + dc):
+    assert helpers.rel_err(a.cpu(), b.cpu()) <= 1e-4
+  need, overflow = r.bin_stats()
+  assert not overflow and need > 0
