@@ -279,7 +279,8 @@ __global__ __launch_bounds__(kBX* kBY) void target_kernel(
   if (grad_mode == 1 && loss) {
     // Wave reduction, then one atomic per wave.
     for (int off = 32; off > 0; off >>= 1) partial += __shfl_xor(partial, off);
-    if (__lane_id() == 0 && partial != 0.0f) unsafeAtomicAdd(loss, partial);
+    if (__lane_id() == 0 && partial != 0.0f)
+      unsafeAtomicAdd(loss, partial * loss_scale);
   }
 }
 
