@@ -46,8 +46,8 @@ class SplatJob(ctypes.Structure):
               ("triangle_ids", _c_ptr), ("z_buffer", _c_ptr),
               ("barycentrics", _c_ptr), ("image", _c_ptr),
               ("target", _c_ptr), ("d_image", _c_ptr),
-              ("grad_scratch", _c_ptr), ("d_colors", _c_ptr),
-              ("d_vertices", _c_ptr), ("num_vertices", _c_int),
+              ("grad_scratch", _c_ptr), ("d_records", _c_ptr),
+              ("num_vertices", _c_int),
               ("num_triangles", _c_int), ("width", _c_int),
               ("height", _c_int), ("tile_begin", _c_i64)]
 
@@ -100,9 +100,11 @@ SIGNATURES = {
                              ctypes.c_float, ctypes.c_float, ctypes.c_float,
                              ctypes.c_float, _c_ptr]),
     "dr_splat_jobs_prepare": (_c_int, [_c_ptr, _c_int, _c_int, _c_ptr]),
+    "dr_render_record_size": (_c_int, [_c_int]),
+    "dr_render_splat_workspace_size": (_c_size, [_c_i64]),
     "dr_render_splat_forward": (_c_int, [_c_ptr, _c_int, _c_i64, _c_int,
                                          _c_int, _c_int, ctypes.c_float,
-                                         _c_ptr, _c_ptr]),
+                                         _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "dr_render_splat_backward": (_c_int, [_c_ptr, _c_int, _c_i64, _c_int,
                                           _c_int, _c_ptr]),
 }

@@ -44,8 +44,16 @@ class BatchedSplatRenderer:
       if m["colors"].shape[1] != C:
         raise ValueError("all meshes need the same channel count")
     self.C = C
-    self.d_vertices = [torch.zeros_like(m["vertices"]) for m in meshes]
-    self.d_colors = [torch.zeros_like(m["colors"]) for m in meshes]
+    # Per-mesh gradient records [V, R]: d position in cols 0..2, d colour in
+    # cols 4..4+C-1 (one contiguous record per vertex for grouped atomics).
+    R = _native.lib().dr_render_record_size(C)
+    if R <= 0:
+      raise ValueError(f"unsupported channel count {C}")
+    self.records = [torch.zeros((m["vertices"].shape[0], R),
+                                dtype=torch.float32, device=dev)
+                    for m in meshes]
+    self.d_vertices = [r[:, 0:3] for r in self.records]
+    self.d_colors = [r[:, 4:4 + C] for r in self.records]
     self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
     self.clip, self.ids, self.z, self.barys = [], [], [], []
     self.images, self.targets, self.scratch = [], [], []
@@ -141,8 +149,7 @@ class BatchedSplatRenderer:
       j.target = self.targets[i].data_ptr()
       j.d_image = None
       j.grad_scratch = self.scratch[i].data_ptr()
-      j.d_colors = self.d_colors[mi].data_ptr()
-      j.d_vertices = self.d_vertices[mi].data_ptr()
+      j.d_records = self.records[mi].data_ptr()
       j.num_vertices = m["vertices"].shape[0]
       j.num_triangles = m["triangles"].shape[0]
       j.width = w
@@ -153,6 +160,8 @@ class BatchedSplatRenderer:
                   "dr_splat_jobs_prepare")
     self.splat_table = _native.table_to_device(table, self.device)
     self.splat_tiles = tiles.value
+    self.splat_ws = _native.workspace(
+        lib.dr_render_splat_workspace_size(tiles.value), self.device)
 
   # -- stages ----------------------------------------------------------------
 
@@ -182,7 +191,8 @@ class BatchedSplatRenderer:
         _native.lib().dr_render_splat_forward(
             _native.ptr(self.splat_table), len(self.jobs), self.splat_tiles,
             self.L, self.C, grad_mode, self.loss_scale,
-            _native.ptr(self.loss), _native.stream_handle()),
+            _native.ptr(self.loss), _native.ptr(self.splat_ws),
+            self.splat_ws.numel(), _native.stream_handle()),
         "dr_render_splat_forward")
 
   def splat_backward(self):
@@ -194,7 +204,7 @@ class BatchedSplatRenderer:
 
   def zero_grads(self):
     # Fill kernels, not memsets, so the step can be graph-captured.
-    for t in self.d_vertices + self.d_colors:
+    for t in self.records:
       t.zero_()
     self.loss.zero_()
 

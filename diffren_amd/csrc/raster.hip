@@ -477,61 +477,110 @@ __device__ __forceinline__ void insert_key(uint64_t* keys, uint32_t* ties,
 // when the list is full: a fragment with larger z is never kept, and until it
 // is pushed out it acts on the smaller ones exactly like an empty slot (both
 // compare greater), so it cannot change which of them survive or their order.
-// Rare (pixels with exact z ties), so it rescans the tile's bin once per
-// candidate to visit candidates in id order without sorting.
+//
+// The whole block gathers the pixel's candidate fragments from the tile's
+// bin in parallel into LDS; one thread orders them by id and replays.  If a
+// pixel has more candidates than the LDS list holds (many fragments with
+// exactly equal z), that thread rescans the bin once per candidate instead.
+constexpr int kMaxCandidates = 64;
+
+__device__ __forceinline__ bool fragment_at(const dr_raster_job& job, int t,
+                                            int ix, int iy, float sx2,
+                                            float sy2, int cull, Fragment& f) {
+  float4 v[3];
+  Box b;
+  if (!load_triangle(job, t, v) ||
+      !triangle_bbox(v, job.width, job.height, b))
+    return false;
+  if (ix < b.left || ix >= b.right || iy < b.bottom || iy >= b.top)
+    return false;
+  int64_t m[9], r[9];
+  fixed_matrix(v, m);
+  adjugate(m, cull, r);
+  return shade_pixel(ix, iy, sx2, sy2, r, v, f);
+}
+
+// Reference insertion of (z, id) into the L-deep list lz/lid.
+__device__ __forceinline__ void reference_insert(float* lz, int* lid, int L,
+                                                 float z, int id) {
+  for (int l = 0; l < L; ++l) {
+    if (z < lz[l]) {
+      const float tz = lz[l]; lz[l] = z; z = tz;
+      const int ti = lid[l]; lid[l] = id; id = ti;
+    }
+    if (z == 1.0f) break;
+  }
+}
+
+// Block-cooperative; every thread of the block must call it.
 __device__ void replay_pixel(const dr_raster_job& job, int ix, int iy,
                              float sx2, float sy2, int cull, bool overflow,
                              uint32_t cnt, uint64_t off,
                              const uint32_t* __restrict__ bins, int L,
-                             uint64_t* slot, int tp) {
+                             uint64_t* slot, int tp, uint64_t* cand,
+                             int* cand_n) {
   const uint64_t last = slot[(L - 1) * tp];
   const uint32_t zmax = last == kEmpty ? 0xffffffffu : uint32_t(last >> 32);
   const int64_t items = overflow ? int64_t(job.num_triangles) : int64_t(cnt);
-  float lz[kMaxLayers];
-  int lid[kMaxLayers];
-  for (int l = 0; l < L; ++l) {
-    lz[l] = 1.0f;
-    lid[l] = -1;
+  if (threadIdx.x == 0) *cand_n = 0;
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < items; i += blockDim.x) {
+    const int t = overflow ? int(i) : int(bins[off + i]);
+    Fragment f;
+    if (!fragment_at(job, t, ix, iy, sx2, sy2, cull, f)) continue;
+    if (z_order(f.z) > zmax) continue;
+    const int k = atomicAdd(cand_n, 1);
+    if (k < kMaxCandidates)
+      cand[k] = (uint64_t(uint32_t(t)) << 32) | __float_as_uint(f.z);
   }
-  int prev = -1;
-  for (;;) {
-    int next = INT32_MAX;
-    float next_z = 1.0f;
-    for (int64_t i = 0; i < items; ++i) {
-      const int t = overflow ? int(i) : int(bins[off + i]);
-      if (t <= prev || t >= next) continue;
-      float4 v[3];
-      Box b;
-      if (!load_triangle(job, t, v) ||
-          !triangle_bbox(v, job.width, job.height, b))
-        continue;
-      if (ix < b.left || ix >= b.right || iy < b.bottom || iy >= b.top)
-        continue;
-      int64_t m[9], r[9];
-      fixed_matrix(v, m);
-      adjugate(m, cull, r);
-      Fragment f;
-      if (!shade_pixel(ix, iy, sx2, sy2, r, v, f)) continue;
-      if (z_order(f.z) > zmax) continue;
-      next = t;
-      next_z = f.z;
-    }
-    if (next == INT32_MAX) break;
-    float zn = next_z;
-    int idn = next;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int n = *cand_n;
+    float lz[kMaxLayers];
+    int lid[kMaxLayers];
     for (int l = 0; l < L; ++l) {
-      if (zn < lz[l]) {
-        const float tz = lz[l]; lz[l] = zn; zn = tz;
-        const int ti = lid[l]; lid[l] = idn; idn = ti;
-      }
-      if (zn == 1.0f) break;
+      lz[l] = 1.0f;
+      lid[l] = -1;
     }
-    prev = next;
+    if (n <= kMaxCandidates) {
+      // Insertion sort by id (upper 32 bits; ids are distinct).
+      for (int a = 1; a < n; ++a) {
+        const uint64_t x = cand[a];
+        int b = a - 1;
+        while (b >= 0 && cand[b] > x) {
+          cand[b + 1] = cand[b];
+          --b;
+        }
+        cand[b + 1] = x;
+      }
+      for (int a = 0; a < n; ++a)
+        reference_insert(lz, lid, L, __uint_as_float(uint32_t(cand[a])),
+                         int(cand[a] >> 32));
+    } else {
+      int prev = -1;
+      for (;;) {
+        int next = INT32_MAX;
+        float next_z = 1.0f;
+        for (int64_t i = 0; i < items; ++i) {
+          const int t = overflow ? int(i) : int(bins[off + i]);
+          if (t <= prev || t >= next) continue;
+          Fragment f;
+          if (!fragment_at(job, t, ix, iy, sx2, sy2, cull, f)) continue;
+          if (z_order(f.z) > zmax) continue;
+          next = t;
+          next_z = f.z;
+        }
+        if (next == INT32_MAX) break;
+        reference_insert(lz, lid, L, next_z, next);
+        prev = next;
+      }
+    }
+    for (int l = 0; l < L; ++l)
+      slot[l * tp] = lid[l] < 0 ? kEmpty
+                                : (uint64_t(z_order(lz[l])) << 32) |
+                                      uint32_t(lid[l]);
   }
-  for (int l = 0; l < L; ++l)
-    slot[l * tp] = lid[l] < 0 ? kEmpty
-                              : (uint64_t(z_order(lz[l])) << 32) |
-                                    uint32_t(lid[l]);
+  __syncthreads();
 }
 
 struct TriRect {
@@ -550,6 +599,8 @@ __global__ __launch_bounds__(kBlock) void raster_tile_kernel(
   __shared__ TriRect queue[kBlock];
   __shared__ int queue_n;
   __shared__ uint32_t ties[(TP + 31) / 32];
+  __shared__ uint64_t cand[kMaxCandidates];
+  __shared__ int cand_n;
 
   const int64_t flat_tile = blockIdx.x;
   const int j = find_job<true>(J, flat_tile);
@@ -635,15 +686,19 @@ __global__ __launch_bounds__(kBlock) void raster_tile_kernel(
   }
   __syncthreads();
 
-  // Exact replay of the reference's insertion order at pixels with z ties.
-  for (int p = threadIdx.x; p < TP; p += kBlock) {
-    if (!(ties[p >> 5] & (1u << (p & 31)))) continue;
-    const int ix = x0 + p % TW, iy = y0 + p / TW;
-    if (ix >= x1 || iy >= y1) continue;
-    replay_pixel(job, ix, iy, sx2, sy2, cull, overflow, cnt, off, bins, L,
-                 keys + p, TP);
+  // Exact replay of the reference's insertion order at pixels with z ties
+  // (block-uniform loop: the flags are read from LDS by every thread).
+  for (int w = 0; w < (TP + 31) / 32; ++w) {
+    uint32_t bits = ties[w];
+    while (bits) {
+      const int p = w * 32 + __builtin_ctz(bits);
+      bits &= bits - 1;
+      const int ix = x0 + p % TW, iy = y0 + p / TW;
+      if (ix >= x1 || iy >= y1) continue;
+      replay_pixel(job, ix, iy, sx2, sy2, cull, overflow, cnt, off, bins, L,
+                   keys + p, TP, cand, &cand_n);
+    }
   }
-  __syncthreads();
 
   // Resolve: recompute the winning fragments exactly and write the outputs.
   const int64_t P = int64_t(W) * H;

@@ -80,18 +80,38 @@ __device__ __forceinline__ float interp3(const Corner& k, float a0, float a1,
   return (k.b[0] * a0 + k.b[1] * a1) + k.b[2] * a2;
 }
 
-// Viewport x, y of pixel-layer q (splat.py:214-221).
-__device__ __forceinline__ void viewport_xy(const dr_splat_job& J,
-                                            const Corner& k, float& x,
-                                            float& y, float& cx, float& cy,
-                                            float& cw) {
+// Offset from a splat to the centre of its own pixel, floor(x) + 0.5 - x,
+// where x = ((c / w + 1) * size) * 0.5 is its viewport coordinate
+// (splat.py:214-221, 103-104).  In rasterize-then-splat x sits on the pixel
+// centre up to rounding, so this offset is pure rounding noise of ~1e-5 px at
+// x ~ 1000 when x is formed in float32 -- yet it is what the tap weights'
+// position gradients are evaluated at.  It is computed here as
+//   -(a c + k w) / w,  a = size / 2, k = a - floor(x) - 0.5 (both exact),
+// with the product k w split exactly by an fma (two-product), so the
+// cancellation happens without rounding and the offset is accurate to a few
+// ulps of itself.  The forward image is unchanged to far below 1e-5.
+__device__ __forceinline__ float center_offset(float c, float w, int size) {
+  const float a = 0.5f * float(size);
+  const float x = ((c / w + 1.0f) * float(size)) * 0.5f;
+  const float k = (a - floorf(x)) - 0.5f;
+  const float p = k * w;
+  const float e = fmaf(k, w, -p);
+  const float num = fmaf(a, c, p) + e;
+  return -(num / w);
+}
+
+// Interpolated clip x, y, w and the splat's pixel-centre offsets.
+__device__ __forceinline__ void splat_position(const dr_splat_job& J,
+                                               const Corner& k, float& ox,
+                                               float& oy, float& cx,
+                                               float& cy, float& cw) {
   const float4* cv = reinterpret_cast<const float4*>(J.clip_vertices);
   const float4 p0 = cv[k.v[0]], p1 = cv[k.v[1]], p2 = cv[k.v[2]];
   cx = interp3(k, p0.x, p1.x, p2.x);
   cy = interp3(k, p0.y, p1.y, p2.y);
   cw = interp3(k, p0.w, p1.w, p2.w);
-  x = ((cx / cw + 1.0f) * float(J.width)) * 0.5f;
-  y = ((cy / cw + 1.0f) * float(J.height)) * 0.5f;
+  ox = center_offset(cx, cw, J.width);
+  oy = center_offset(cy, cw, J.height);
 }
 
 __device__ __forceinline__ float viewport_z(float z) {
@@ -117,11 +137,13 @@ __device__ __forceinline__ int layer_class(int surface, int l) {
   return surface > l ? 0 : (surface == l ? 1 : 2);
 }
 
-__device__ __forceinline__ float tap_weight(float coef, float sx, float sy,
+// Tap weight for offset d = (ox + u, oy + v) (splat.py:106-115; the
+// foreground mask is 1 for every staged splat).
+__device__ __forceinline__ float tap_weight(float coef, float ox, float oy,
                                             int u, int v, float& dx,
                                             float& dy) {
-  dx = ((floorf(sx) + 0.5f) - sx) + float(u);
-  dy = ((floorf(sy) + 0.5f) - sy) + float(v);
+  dx = ox + float(u);
+  dy = oy + float(v);
   const float d2 = dx * dx + dy * dy;
   return (1.0f * coef) * expf(-2.0f * d2);
 }
@@ -130,13 +152,14 @@ __device__ __forceinline__ float norm_scale(float w) {
   return 1.0f / (fmaxf(w - 1.0f, 0.0f) + 1.0f);
 }
 
-// LDS stage, structure of arrays over [L][kRegion]: x, y, z, colour[C].
-// Background / outside-the-image entries have x = NaN as the mask (the fg
-// mask is 1 exactly where the rasterizer stored a triangle).
+// LDS stage, structure of arrays over [L][kRegion]: pixel-centre offsets
+// x, y, viewport z, colour[C].  Background / outside-the-image entries have
+// x = NaN as the mask (the fg mask is 1 exactly where the rasterizer stored a
+// triangle).
 template <int C>
 __global__ __launch_bounds__(kBX* kBY) void target_kernel(
     const dr_splat_job* __restrict__ jobs, int num_jobs, int L, float coef,
-    int grad_mode, float loss_scale, float* __restrict__ loss) {
+    int grad_mode, float loss_scale, float* __restrict__ loss_partials) {
   extern __shared__ float lds[];
   const int R = kRegion;
   float* sx = lds;
@@ -166,7 +189,7 @@ __global__ __launch_bounds__(kBX* kBY) void target_kernel(
       Corner k;
       if (corners(J, q, k)) {
         float cx, cy, cw;
-        viewport_xy(J, k, x, y, cx, cy, cw);
+        splat_position(J, k, x, y, cx, cy, cw);
         const float* a0 = J.colors + int64_t(k.v[0]) * C;
         const float* a1 = J.colors + int64_t(k.v[1]) * C;
         const float* a2 = J.colors + int64_t(k.v[2]) * C;
@@ -276,17 +299,55 @@ __global__ __launch_bounds__(kBX* kBY) void target_kernel(
       }
     }
   }
-  if (grad_mode == 1 && loss) {
-    // Wave reduction, then one atomic per wave.
+  if (grad_mode == 1) {
+    // Tile partial of the loss: wave shuffles, then the block's waves.
+    __shared__ float wave_sum[kBX * kBY / 64];
     for (int off = 32; off > 0; off >>= 1) partial += __shfl_xor(partial, off);
-    if (__lane_id() == 0 && partial != 0.0f)
-      unsafeAtomicAdd(loss, partial * loss_scale);
+    if (__lane_id() == 0) wave_sum[tid >> 6] = partial;
+    __syncthreads();
+    if (tid == 0) {
+      float t = 0.0f;
+      for (int w = 0; w < kBX * kBY / 64; ++w) t += wave_sum[w];
+      loss_partials[tile] = t * loss_scale;
+    }
   }
 }
 
+// Sums the per-tile loss partials in a fixed order (one block) into *loss.
+__global__ __launch_bounds__(1024) void loss_reduce_kernel(
+    const float* __restrict__ partials, int64_t n, float* __restrict__ loss) {
+  __shared__ float red[1024];
+  float t = 0.0f;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) t += partials[i];
+  red[threadIdx.x] = t;
+  __syncthreads();
+  for (int d = 512; d > 0; d >>= 1) {
+    if (int(threadIdx.x) < d) red[threadIdx.x] += red[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *loss += red[0];
+}
+
+// Gradient record of one vertex: [R] floats, R = record_size(C):
+// cols 0..3 d position (x, y, z, 0) -- or d clip (x, y, 0, w) when the job
+// has no camera -- and cols 4 .. 4 + C - 1 d colour.
+template <int C>
+struct Record {
+  static constexpr int kSize = (4 + C) <= 8 ? 8 : ((4 + C) <= 16 ? 16 : 32);
+};
+
+// Source pass.  Float atomics are the cost of this pass; they run ~7x faster
+// when the lanes of one atomic instruction cover contiguous records than
+// when each lane hits its own scattered address, so each wave stages its
+// fragments' three vertex records in LDS and flushes them with kSize
+// consecutive lanes per record.
 template <int C>
 __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     const dr_splat_job* __restrict__ jobs, int num_jobs, int L, float coef) {
+  constexpr int R = Record<C>::kSize;
+  constexpr int kPerWave = 3 * 64;             // entries per wave and layer
+  __shared__ float stage_val[kBX * kBY / 64][kPerWave][R];
+  __shared__ int stage_vtx[kBX * kBY / 64][kPerWave];
   const int64_t tile = blockIdx.x;
   const dr_splat_job& J = jobs[find_job(jobs, num_jobs, tile)];
   const int W = J.width, H = J.height;
@@ -294,103 +355,128 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   const int local = int(tile - J.tile_begin);
   const int x = (local % ntx) * kBX + threadIdx.x;
   const int y = (local / ntx) * kBY + threadIdx.y;
-  if (x >= W || y >= H) return;
+  const int lane = __lane_id();
+  const int wave = (threadIdx.y * kBX + threadIdx.x) >> 6;
+  float(*val)[R] = stage_val[wave];
+  int* vtx = stage_vtx[wave];
+  const bool inside = x < W && y < H;
   const int64_t P = int64_t(W) * H;
   const int64_t p = int64_t(y) * W + x;
 
-  // Skip pixels with no foreground layer before touching the neighbours.
   bool any = false;
-  for (int l = 0; l < L; ++l) any |= J.triangle_ids[l * P + p] >= 0;
-  if (!any) return;
+  if (inside)
+    for (int l = 0; l < L; ++l) any |= J.triangle_ids[l * P + p] >= 0;
 
   float zS[kMaxLayers], zT[kMaxLayers];
-  for (int l = 0; l < L; ++l) zS[l] = viewport_z(J.z_buffer[l * P + p]);
   int surf[9];
-#pragma unroll
-  for (int t = 0; t < 9; ++t) {
-    const int u = t / 3 - 1, v = t % 3 - 1;
-    const int tx = x + u, ty = y + v;
-    surf[t] = 0;
-    if (tx < 0 || tx >= W || ty < 0 || ty >= H) continue;
-    const int64_t pt = int64_t(ty) * W + tx;
-    for (int l = 0; l < L; ++l) zT[l] = viewport_z(J.z_buffer[l * P + pt]);
-    surf[t] = surface_index(zS, zT, L);
-  }
-  for (int l = 0; l < L; ++l) {
-    const int64_t q = l * P + p;
-    Corner k;
-    if (!corners(J, q, k)) continue;
-    float sx, sy, cx, cy, cw;
-    viewport_xy(J, k, sx, sy, cx, cy, cw);
-    const float* a0 = J.colors + int64_t(k.v[0]) * C;
-    const float* a1 = J.colors + int64_t(k.v[1]) * C;
-    const float* a2 = J.colors + int64_t(k.v[2]) * C;
-    float col[C], dcol[C];
-#pragma unroll
-    for (int c = 0; c < C; ++c) {
-      col[c] = interp3(k, a0[c], a1[c], a2[c]);
-      dcol[c] = 0.0f;
-    }
-    float gx = 0.0f, gy = 0.0f;
+  if (any) {
+    for (int l = 0; l < L; ++l) zS[l] = viewport_z(J.z_buffer[l * P + p]);
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
       const int u = t / 3 - 1, v = t % 3 - 1;
       const int tx = x + u, ty = y + v;
+      surf[t] = 0;
       if (tx < 0 || tx >= W || ty < 0 || ty >= H) continue;
-      float dx, dy;
-      const float w = tap_weight(coef, sx, sy, u, v, dx, dy);
-      const int cls = layer_class(surf[t], l);
-      const float* gp =
-          J.grad_scratch + ((int64_t(ty) * W + tx) * 3 + cls) * (C + 1);
-      float dw = gp[C];
+      const int64_t pt = int64_t(ty) * W + tx;
+      for (int l = 0; l < L; ++l) zT[l] = viewport_z(J.z_buffer[l * P + pt]);
+      surf[t] = surface_index(zS, zT, L);
+    }
+  }
+  // Every lane of the wave runs every layer iteration (ballots below).
+  for (int l = 0; l < L; ++l) {
+    Corner k;
+    const bool frag = any && corners(J, l * P + p, k);
+    float rec[3][R];
+    if (frag) {
+      const int64_t q = l * P + p;
+      (void)q;
+      float sx, sy, cx, cy, cw;
+      splat_position(J, k, sx, sy, cx, cy, cw);
+      const float* a0 = J.colors + int64_t(k.v[0]) * C;
+      const float* a1 = J.colors + int64_t(k.v[1]) * C;
+      const float* a2 = J.colors + int64_t(k.v[2]) * C;
+      float col[C], dcol[C];
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        dcol[c] += w * gp[c];
-        dw += gp[c] * col[c];
+        col[c] = interp3(k, a0[c], a1[c], a2[c]);
+        dcol[c] = 0.0f;
       }
-      const float dw4 = dw * 4.0f * w;
-      gx += dw4 * dx;
-      gy += dw4 * dy;
-    }
-    // x = ((cx / cw + 1) W) / 2  ->  d cx = gx W / (2 cw),
-    // d cw = -(gx W cx + gy H cy) / (2 cw^2).
-    const float hx = gx * (0.5f * float(W)), hy = gy * (0.5f * float(H));
-    const float inv_w = 1.0f / cw;
-    const float dcx = hx * inv_w, dcy = hy * inv_w;
-    const float dcw = -(dcx * cx + dcy * cy) * inv_w;
-    if (J.d_colors) {
+      float gx = 0.0f, gy = 0.0f;
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        float* d = J.d_colors + int64_t(k.v[i]) * C;
+      for (int t = 0; t < 9; ++t) {
+        const int u = t / 3 - 1, v = t % 3 - 1;
+        const int tx = x + u, ty = y + v;
+        if (tx < 0 || tx >= W || ty < 0 || ty >= H) continue;
+        float dx, dy;
+        const float w = tap_weight(coef, sx, sy, u, v, dx, dy);
+        const int cls = layer_class(surf[t], l);
+        const float* gp =
+            J.grad_scratch + ((int64_t(ty) * W + tx) * 3 + cls) * (C + 1);
+        float dw = gp[C];
 #pragma unroll
-        for (int c = 0; c < C; ++c)
-          if (dcol[c] != 0.0f) unsafeAtomicAdd(d + c, k.b[i] * dcol[c]);
+        for (int c = 0; c < C; ++c) {
+          dcol[c] += w * gp[c];
+          dw += gp[c] * col[c];
+        }
+        const float dw4 = dw * 4.0f * w;
+        gx += dw4 * dx;
+        gy += dw4 * dy;
       }
-    }
-    if (J.d_vertices) {
+      // The offsets are -(x - centre) with x = ((cx / cw + 1) W) / 2, so
+      // d cx = gx W / (2 cw), d cw = -(gx W cx + gy H cy) / (2 cw^2) with
+      // (gx, gy) the gradient with respect to x, y.
+      const float hx = gx * (0.5f * float(W)), hy = gy * (0.5f * float(H));
+      const float inv_w = 1.0f / cw;
+      const float dcx = hx * inv_w, dcy = hy * inv_w;
+      const float dcw = -(dcx * cx + dcy * cy) * inv_w;
+      float dp[4];
       if (J.camera) {
         // d v_j = sum_i M[i][j] d clip_i over i in {x, y, w} (z gets none).
         const float* M = J.camera;
-        float dv[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-          dv[j] = (M[j] * dcx + M[4 + j] * dcy) + M[12 + j] * dcw;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          float* d = J.d_vertices + int64_t(k.v[i]) * 3;
-#pragma unroll
-          for (int j = 0; j < 3; ++j) unsafeAtomicAdd(d + j, k.b[i] * dv[j]);
-        }
+          dp[j] = (M[j] * dcx + M[4 + j] * dcy) + M[12 + j] * dcw;
+        dp[3] = 0.0f;
       } else {
+        dp[0] = dcx; dp[1] = dcy; dp[2] = 0.0f; dp[3] = dcw;
+      }
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          float* d = J.d_vertices + int64_t(k.v[i]) * 4;
-          unsafeAtomicAdd(d + 0, k.b[i] * dcx);
-          unsafeAtomicAdd(d + 1, k.b[i] * dcy);
-          unsafeAtomicAdd(d + 3, k.b[i] * dcw);
+      for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) rec[i][c] = k.b[i] * dp[c];
+#pragma unroll
+        for (int c = 0; c < C; ++c) rec[i][4 + c] = k.b[i] * dcol[c];
+#pragma unroll
+        for (int c = 4 + C; c < R; ++c) rec[i][c] = 0.0f;
+      }
+    }
+    // Wave-local append of 3 entries per fragment, then a grouped flush.
+    const uint64_t act = __ballot(frag);
+    if (act == 0) continue;
+    if (frag) {
+      const int base = 3 * __popcll(act & ((uint64_t(1) << lane) - 1));
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        vtx[base + i] = k.v[i];
+#pragma unroll
+        for (int c = 0; c < R; ++c) val[base + i][c] = rec[i][c];
+      }
+    }
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+    const int n = 3 * __popcll(act);
+    float* grads = J.d_records;
+    if (grads) {
+      for (int e0 = 0; e0 < n; e0 += 64 / R) {
+        const int e = e0 + lane / R, c = lane % R;
+        if (e < n) {
+          const float g = val[e][c];
+          if (g != 0.0f) unsafeAtomicAdd(grads + int64_t(vtx[e]) * R + c, g);
         }
       }
     }
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -408,7 +494,7 @@ float splat_coef() {
 template <int C>
 int launch_target(const dr_splat_job* jobs, int n, int64_t tiles, int L,
                   int grad_mode, float loss_scale, float* loss,
-                  hipStream_t s) {
+                  float* partials, hipStream_t s) {
   const size_t lds = target_lds_bytes(L, C);
   if (lds > 65536) {
     DR_CHECK_HIP(hipFuncSetAttribute(
@@ -416,8 +502,12 @@ int launch_target(const dr_splat_job* jobs, int n, int64_t tiles, int L,
         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
   }
   target_kernel<C><<<unsigned(tiles), dim3(kBX, kBY), lds, s>>>(
-      jobs, n, L, splat_coef(), grad_mode, loss_scale, loss);
+      jobs, n, L, splat_coef(), grad_mode, loss_scale, partials);
   DR_CHECK_LAUNCH("render splat target pass");
+  if (grad_mode == 1 && loss) {
+    loss_reduce_kernel<<<1, 1024, 0, s>>>(partials, tiles, loss);
+    DR_CHECK_LAUNCH("render splat loss reduce");
+  }
   return kOk;
 }
 
@@ -486,19 +576,34 @@ int dr_splat_jobs_prepare(dr_splat_job* jobs, int32_t n, int32_t channels,
   return DR_OK;
 }
 
+int32_t dr_render_record_size(int32_t channels) {
+  if (channels < 1 || channels > dr::kMaxChannels) return 0;
+  return (4 + channels) <= 8 ? 8 : ((4 + channels) <= 16 ? 16 : 32);
+}
+
+size_t dr_render_splat_workspace_size(int64_t total_tiles) {
+  return size_t(total_tiles > 0 ? total_tiles : 1) * sizeof(float);
+}
+
 int dr_render_splat_forward(const dr_splat_job* jobs_device, int32_t num_jobs,
                             int64_t total_tiles, int32_t num_layers,
                             int32_t channels, int32_t grad_mode,
-                            float loss_scale, float* loss,
-                            dr_stream_t stream) {
+                            float loss_scale, float* loss, void* workspace,
+                            size_t workspace_bytes, dr_stream_t stream) {
   int rc = dr::check(num_jobs, total_tiles, num_layers, channels);
   if (rc != DR_OK) return rc;
   DR_CHECK_ARG(jobs_device != nullptr, "jobs is null");
   DR_CHECK_ARG(grad_mode >= 0 && grad_mode <= 2, "grad_mode must be 0, 1, 2");
+  if (grad_mode == 1) {
+    DR_CHECK_ARG(workspace != nullptr, "workspace is null");
+    if (workspace_bytes < dr_render_splat_workspace_size(total_tiles))
+      return dr::fail(dr::kWorkspaceTooSmall, "render workspace too small");
+  }
   hipStream_t s = static_cast<hipStream_t>(stream);
+  float* partials = static_cast<float*>(workspace);
 #define DR_CALL(CC)                                                     \
   dr::launch_target<CC>(jobs_device, num_jobs, total_tiles, num_layers, \
-                        grad_mode, loss_scale, loss, s)
+                        grad_mode, loss_scale, loss, partials, s)
   DR_RENDER_SWITCH(channels, DR_CALL)
 #undef DR_CALL
   return dr::fail(dr::kUnsupported, "unsupported channel count");

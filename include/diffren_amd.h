@@ -227,9 +227,12 @@ typedef struct dr_splat_job {
   const float* target;          /* [H, W, C] L2 target (grad_mode 1) */
   const float* d_image;         /* [H, W, C] upstream gradient (grad_mode 2) */
   float* grad_scratch;          /* [H, W, 3, C + 1] per-job workspace */
-  float* d_colors;              /* [V, C] accumulated, or NULL */
-  float* d_vertices;            /* [V, 3] accumulated (M^T d clip) when camera
-                                   is set, else [V, 4] d clip; or NULL */
+  float* d_records;             /* [V, R] accumulated gradient records, or
+                                   NULL; R = dr_render_record_size(C).
+                                   cols 0-2: d vertex = M^T d clip when
+                                   camera is set, else cols 0, 1, 3: d clip
+                                   x, y, w; cols 4 .. 4+C-1: d colours.
+                                   Jobs that share a mesh share it. */
   int32_t num_vertices;
   int32_t num_triangles;
   int32_t width;
@@ -241,23 +244,32 @@ typedef struct dr_splat_job {
 int dr_splat_jobs_prepare(dr_splat_job* jobs_host, int32_t num_jobs,
                           int32_t channels, int64_t* total_tiles);
 
+/* Floats per vertex gradient record for `channels` colour channels (8, 16
+ * or 32; 0 if unsupported). */
+int32_t dr_render_record_size(int32_t channels);
+
+/* Workspace bytes of dr_render_splat_forward (per-tile loss partials). */
+size_t dr_render_splat_workspace_size(int64_t total_tiles);
+
 /* Target pass over all jobs in one launch: interpolate colours and viewport
  * positions (interpolate.py:24-63, splat.py:209-221), splat
  * (splat.py:26-175) and composite (composite.py:20-37) into `image`.
  * grad_mode 0: image only.  1: also the L2 loss sum((image - target)^2) *
- * loss_scale accumulated into *loss (device float, caller clears) and its
- * gradient 2 (image - target) loss_scale pushed into grad_scratch.  2: the
- * given d_image pushed into grad_scratch.  channels 1-8, 12 or 16.
+ * loss_scale added to *loss (device float, caller clears; summed per tile,
+ * then in a fixed order, so it is deterministic) and its gradient
+ * 2 (image - target) loss_scale pushed into grad_scratch.  2: the given
+ * d_image pushed into grad_scratch.  channels 1-8, 12 or 16.  The workspace
+ * is needed for grad_mode 1 only.
  * No reference counterpart as one call: it fuses render.py:123-132 for an
  * identity shader, batched where the reference runs one launch per image. */
 int dr_render_splat_forward(const dr_splat_job* jobs_device, int32_t num_jobs,
                             int64_t total_tiles, int32_t num_layers,
                             int32_t channels, int32_t grad_mode,
-                            float loss_scale, float* loss,
-                            dr_stream_t stream);
+                            float loss_scale, float* loss, void* workspace,
+                            size_t workspace_bytes, dr_stream_t stream);
 
 /* Source pass: from grad_scratch (written by a forward with grad_mode 1 or
- * 2), accumulate d_colors and d_vertices for all jobs (float atomics).  The
+ * 2), accumulate the d_records of all jobs (float atomics).  The
  * hand-written VJP of splat.py:178-229 + interpolate.py:24-63 +
  * transforms.py:40-78; the reference uses XLA autodiff. */
 int dr_render_splat_backward(const dr_splat_job* jobs_device,
