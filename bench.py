@@ -197,10 +197,14 @@ def main():
   w["targets"] = [img.clone() for img in tr.images]
   del tr
 
-  r = batch.BatchedSplatRenderer([w["mesh"]], jobs, L)
+  r = batch.BatchedSplatRenderer([w["mesh"]], jobs, L, tile="auto")
+  log(f"autotuned raster tile {r.raster_config.tile_w}x"
+      f"{r.raster_config.tile_h}, bin capacity {r.raster_config.bin_capacity}"
+      f", timings {r.autotune_timings}")
   for i in range(B):
     r.targets[i].copy_(w["targets"][i])
-  entries, overflow = (r.step(), r.bin_stats())[1]
+  r.step()
+  entries, overflow = r.bin_stats()
   log(f"bin entries {entries} overflow {overflow}")
 
   use_graph = not args.no_graph
@@ -243,8 +247,8 @@ def main():
                        " grad)", bytes_target),
       "splat_source": ("render_splat source pass (splat/interpolate/"
                        "transform VJP)", bytes_source),
-      "rasterize": ("tile rasterizer (bin + tile raster + resolve)",
-                    bytes_raster),
+      "rasterize": ("tile rasterizer (bin count + scan + bin fill + tile "
+                    "raster/resolve)", bytes_raster),
   }
   dom = max(per_stage, key=lambda k: stage_ms[k])
   name, nbytes = per_stage[dom]

@@ -70,6 +70,9 @@ SIGNATURES = {
     "dr_raster_bin_stats": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr]),
     "dr_transform_homogeneous": (_c_int, [_c_ptr, _c_ptr, _c_int, _c_i64,
                                           _c_ptr, _c_ptr]),
+    "dr_transform_homogeneous_multi": (_c_int, [_c_ptr, _c_int, _c_ptr,
+                                                _c_int, _c_i64, _c_ptr,
+                                                _c_ptr]),
     "dr_transform_homogeneous_bwd": (_c_int, [_c_ptr, _c_ptr, _c_int, _c_i64,
                                               _c_ptr, _c_ptr]),
     "dr_interpolate_fwd": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_int, _c_ptr,

@@ -57,10 +57,24 @@ class BatchedSplatRenderer:
     self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
     self.clip, self.ids, self.z, self.barys = [], [], [], []
     self.images, self.targets, self.scratch = [], [], []
-    npix = 0
-    for mi, cam, w, h in jobs:
+    # Clip-space vertices: one [views, V, 4] block per mesh, one transform
+    # launch per mesh.
+    self.mesh_views = {}
+    for i, (mi, _, _, _) in enumerate(jobs):
+      self.mesh_views.setdefault(mi, []).append(i)
+    self.clip_blocks, self.camera_blocks = {}, {}
+    for mi, views in self.mesh_views.items():
       V = meshes[mi]["vertices"].shape[0]
-      self.clip.append(torch.empty((V, 4), dtype=torch.float32, device=dev))
+      self.clip_blocks[mi] = torch.empty((len(views), V, 4),
+                                         dtype=torch.float32, device=dev)
+      self.camera_blocks[mi] = torch.stack(
+          [jobs[i][1].to(torch.float32).reshape(4, 4) for i in views])
+    slot = {}
+    for i, (mi, _, _, _) in enumerate(jobs):
+      slot[i] = self.mesh_views[mi].index(i)
+    npix = 0
+    for i, (mi, cam, w, h) in enumerate(jobs):
+      self.clip.append(self.clip_blocks[mi][slot[i]])
       self.ids.append(torch.empty((self.L, h, w), dtype=torch.int32,
                                   device=dev))
       self.z.append(torch.empty((self.L, h, w), dtype=torch.float32,
@@ -168,13 +182,13 @@ class BatchedSplatRenderer:
   def transform(self):
     lib = _native.lib()
     s = _native.stream_handle()
-    for i, (mi, _, _, _) in enumerate(self.jobs):
+    for mi, views in self.mesh_views.items():
       v = self.meshes[mi]["vertices"]
       _native.check(
-          lib.dr_transform_homogeneous(_native.ptr(self.cameras[i]),
-                                       _native.ptr(v), v.shape[1],
-                                       v.shape[0], _native.ptr(self.clip[i]),
-                                       s), "dr_transform_homogeneous")
+          lib.dr_transform_homogeneous_multi(
+              _native.ptr(self.camera_blocks[mi]), len(views), _native.ptr(v),
+              v.shape[1], v.shape[0], _native.ptr(self.clip_blocks[mi]), s),
+          "dr_transform_homogeneous_multi")
 
   def rasterize(self):
     lib = _native.lib()

@@ -126,6 +126,25 @@ __global__ __launch_bounds__(kBlock) void transform_bwd_kernel(
                      M[12 + j] * g.w;
 }
 
+// One mesh through `views` matrices [views][4][4]: out [views][V][4].
+__global__ __launch_bounds__(kBlock) void transform_multi_kernel(
+    const float* __restrict__ Ms, const float* __restrict__ in, int32_t D,
+    int64_t V, float* __restrict__ out) {
+  const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (v >= V) return;
+  const float* M = Ms + 16 * blockIdx.y;
+  const float x = in[v * D], y = in[v * D + 1], z = in[v * D + 2];
+  const float w = D == 4 ? in[v * D + 3] : 1.0f;
+  float4 r;
+  float* rp = &r.x;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float* m = M + 4 * i;
+    rp[i] = ((m[0] * x + m[1] * y) + m[2] * z) + m[3] * w;
+  }
+  reinterpret_cast<float4*>(out)[int64_t(blockIdx.y) * V + v] = r;
+}
+
 inline unsigned blocks_for(int64_t n) {
   return unsigned((n + kBlock - 1) / kBlock);
 }
@@ -198,6 +217,25 @@ int dr_transform_homogeneous(const float* matrix, const float* vertices,
                          static_cast<hipStream_t>(stream)>>>(
       matrix, vertices, dims, num_vertices, out);
   DR_CHECK_LAUNCH("transform");
+  return DR_OK;
+}
+
+int dr_transform_homogeneous_multi(const float* matrices, int32_t views,
+                                   const float* vertices, int32_t dims,
+                                   int64_t num_vertices, float* out,
+                                   dr_stream_t stream) {
+  DR_CHECK_ARG(dims == 3 || dims == 4, "vertices must be xyz or xyzw");
+  DR_CHECK_ARG(num_vertices >= 0 && views >= 0 && views <= 65535,
+               "bad sizes");
+  if (num_vertices == 0 || views == 0) return DR_OK;
+  DR_CHECK_ARG(matrices && vertices && out, "null pointer");
+  DR_CHECK_ARG((reinterpret_cast<uintptr_t>(out) & 15) == 0,
+               "out must be 16-byte aligned");
+  dr::transform_multi_kernel<<<dim3(dr::blocks_for(num_vertices), views),
+                               dr::kBlock, 0,
+                               static_cast<hipStream_t>(stream)>>>(
+      matrices, vertices, dims, num_vertices, out);
+  DR_CHECK_LAUNCH("transform multi");
   return DR_OK;
 }
 

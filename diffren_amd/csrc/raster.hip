@@ -298,37 +298,73 @@ __global__ void clear_kernel(uint32_t* p, int64_t n) {
 }
 
 
-// Pass 1 (kFill == false): count entries per tile into counter[].
-// Pass 2 (kFill == true): append triangle indices to their tiles' bins, using
-// counter[] as per-tile cursors and offsets[] from the scan.
+// Pass 1 (kFill == false): count entries per tile into counter[] and record
+// each triangle's tile range in desc[]:
+//   desc.x = first tile (flat over all jobs),
+//   desc.y = nx | ny << 8 | tiles-per-row << 16 when nx, ny < 256,
+//            0 when the triangle covers no tile, 0xffffffff to recompute.
+// Pass 2 (kFill == true): append triangle indices to their tiles' bins from
+// desc[] (no vertex gathers or fixed-point set-up), using counter[] as
+// per-tile cursors and offsets[] from the scan.
 // Lanes of a wave that hit the same tile share one atomic (ballot, then ranks
 // from popcounts), which matters because neighbouring triangles of a mesh
 // mostly land in the same tile.
+constexpr uint32_t kRecompute = 0xffffffffu;
+
+template <int TW, int TH>
+__device__ __forceinline__ void tile_range(const Jobs& J, int64_t flat,
+                                           int cull, int& t, int& n, int& nx,
+                                           int& ntx, uint32_t& tile0) {
+  const int j = find_job<false>(J, flat);
+  const dr_raster_job& job = J.table[j];
+  t = int(flat - job.triangle_begin);
+  n = 0;
+  float4 v[3];
+  Box b;
+  if (load_triangle(job, t, v) &&
+      triangle_bbox(v, job.width, job.height, b)) {
+    int64_t m[9];
+    fixed_matrix(v, m);
+    if (!cannot_cover(m, cull)) {
+      const int tx0 = b.left / TW, ty0 = b.bottom / TH;
+      nx = (b.right - 1) / TW - tx0 + 1;
+      n = nx * ((b.top - 1) / TH - ty0 + 1);
+      ntx = tiles_x(job, TW);
+      tile0 = uint32_t(job.tile_begin + int64_t(ty0) * ntx + tx0);
+    }
+  }
+}
+
 template <int TW, int TH, bool kFill>
 __global__ __launch_bounds__(kBlock) void bin_kernel(
     Jobs J, int64_t total_tris, int cull, uint32_t* __restrict__ counter,
     const uint32_t* __restrict__ offsets, uint32_t* __restrict__ bins,
-    uint64_t capacity) {
+    uint64_t capacity, uint2* __restrict__ desc) {
   const int64_t flat = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  int n = 0, tx0 = 0, ty0 = 0, nx = 1, t = 0, ntx = 1;
-  int64_t tile_base = 0;
+  int n = 0, nx = 1, t = 0, ntx = 1;
+  uint32_t tile0 = 0;
   if (flat < total_tris) {
-    const int j = find_job<false>(J, flat);
-    const dr_raster_job& job = J.table[j];
-    t = int(flat - job.triangle_begin);
-    float4 v[3];
-    Box b;
-    if (load_triangle(job, t, v) &&
-        triangle_bbox(v, job.width, job.height, b)) {
-      int64_t m[9];
-      fixed_matrix(v, m);
-      if (!cannot_cover(m, cull)) {
-        tx0 = b.left / TW;
-        ty0 = b.bottom / TH;
-        nx = (b.right - 1) / TW - tx0 + 1;
-        n = nx * ((b.top - 1) / TH - ty0 + 1);
-        tile_base = job.tile_begin;
-        ntx = tiles_x(job, TW);
+    if (!kFill) {
+      tile_range<TW, TH>(J, flat, cull, t, n, nx, ntx, tile0);
+      const int ny = n > 0 ? n / nx : 0;
+      uint32_t shape = 0;
+      if (n > 0)
+        shape = (nx < 256 && ny < 256 && ntx < 65536)
+                    ? uint32_t(nx) | (uint32_t(ny) << 8) |
+                          (uint32_t(ntx) << 16)
+                    : kRecompute;
+      desc[flat] = make_uint2(tile0, shape);
+    } else {
+      const uint2 d = desc[flat];
+      if (d.y == kRecompute) {
+        tile_range<TW, TH>(J, flat, cull, t, n, nx, ntx, tile0);
+      } else if (d.y != 0) {
+        const int j = find_job<false>(J, flat);
+        t = int(flat - J.table[j].triangle_begin);
+        nx = int(d.y & 0xff);
+        n = nx * int((d.y >> 8) & 0xff);
+        ntx = int(d.y >> 16);
+        tile0 = d.x;
       }
     }
   }
@@ -336,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void bin_kernel(
   for (int k = 0; __ballot(k < n); ++k) {
     const bool has = k < n;
     const uint32_t tile =
-        has ? uint32_t(tile_base + int64_t(ty0 + k / nx) * ntx + (tx0 + k % nx))
+        has ? tile0 + uint32_t(k / nx) * uint32_t(ntx) + uint32_t(k % nx)
             : 0xffffffffu;
     uint64_t todo = __ballot(has);
     while (todo) {
@@ -740,6 +776,7 @@ struct Workspace {
   uint32_t* block_sums;
   uint32_t* status;
   uint32_t* bins;
+  uint2* desc;         // per-triangle tile ranges (count pass -> fill pass)
   dr_raster_job* job;  // the single-job entry point's one-entry table
   size_t bytes;
 };
@@ -756,7 +793,8 @@ inline int64_t scan_blocks(int64_t total_tiles) {
   return (total_tiles + kScanTile - 1) / kScanTile;
 }
 
-Workspace carve(void* base, int64_t total_tiles, uint64_t capacity) {
+Workspace carve(void* base, int64_t total_tiles, uint64_t capacity,
+               int64_t total_tris) {
   Workspace w;
   char* p = static_cast<char*>(base);
   size_t o = 0;
@@ -770,6 +808,8 @@ Workspace carve(void* base, int64_t total_tiles, uint64_t capacity) {
   o += align256(sizeof(uint32_t) * scan_blocks(total_tiles));
   w.bins = reinterpret_cast<uint32_t*>(p + o);
   o += align256(sizeof(uint32_t) * capacity);
+  w.desc = reinterpret_cast<uint2*>(p + o);
+  o += align256(sizeof(uint2) * size_t(total_tris > 0 ? total_tris : 1));
   w.job = reinterpret_cast<dr_raster_job*>(p + o);
   o += align256(sizeof(dr_raster_job));
   w.bytes = o;
@@ -793,7 +833,7 @@ int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
   const unsigned tri_blocks = unsigned((total_tris + kBlock - 1) / kBlock);
   if (tri_blocks > 0) {
     bin_kernel<TW, TH, false><<<tri_blocks, kBlock, 0, s>>>(
-        J, total_tris, cull, w.counts, w.offsets, w.bins, capacity);
+        J, total_tris, cull, w.counts, w.offsets, w.bins, capacity, w.desc);
     DR_CHECK_LAUNCH("raster bin count");
   }
   const int64_t sb = scan_blocks(total_tiles);
@@ -810,7 +850,7 @@ int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
   DR_CHECK_LAUNCH("raster clear cursors");
   if (tri_blocks > 0) {
     bin_kernel<TW, TH, true><<<tri_blocks, kBlock, 0, s>>>(
-        J, total_tris, cull, w.counts, w.offsets, w.bins, capacity);
+        J, total_tris, cull, w.counts, w.offsets, w.bins, capacity, w.desc);
     DR_CHECK_LAUNCH("raster bin fill");
   }
   const size_t lds = sizeof(uint64_t) * L * TW * TH;
@@ -840,7 +880,7 @@ int dispatch(const dr_raster_job* table, const dr_raster_job* single,
   DR_CHECK_ARG(ws != nullptr, "workspace is null");
   DR_CHECK_ARG((reinterpret_cast<uintptr_t>(ws) & 255) == 0,
                "workspace must be 256-byte aligned");
-  const Workspace w = carve(ws, total_tiles, capacity);
+  const Workspace w = carve(ws, total_tiles, capacity, total_tris);
   if (ws_bytes < w.bytes)
     return fail(kWorkspaceTooSmall,
                 "workspace too small: need " + std::to_string(w.bytes) +
@@ -910,7 +950,8 @@ size_t dr_raster_workspace_size(int64_t total_triangles, int64_t total_tiles,
                                 const dr_raster_config* config) {
   const dr_raster_config c = dr::resolve_config(config);
   return dr::carve(nullptr, total_tiles,
-                   dr::capacity_of(c, total_triangles, total_tiles))
+                   dr::capacity_of(c, total_triangles, total_tiles),
+                   total_triangles)
       .bytes;
 }
 

@@ -270,7 +270,10 @@ __global__ __launch_bounds__(kBX* kBY) void target_kernel(
         g[c] = J.d_image[p * C + c];
       }
     }
-    if (grad_mode != 0) {
+    // A pixel that received no splat weight is never read by the source
+    // pass (every foreground source adds a positive weight to each of its
+    // 9 targets), so its gradient scratch is left unwritten.
+    if (grad_mode != 0 && (acc[0][C] + acc[1][C]) + acc[2][C] > 0.0f) {
       float dn[3][C];
       float da0 = 0.0f, da1 = 0.0f;
 #pragma unroll

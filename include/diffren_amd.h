@@ -141,6 +141,13 @@ int dr_transform_homogeneous(const float* matrix, const float* vertices,
                              int32_t dims, int64_t num_vertices, float* out,
                              dr_stream_t stream);
 
+/* One mesh through `views` matrices [views, 4, 4]: out [views, V, 4], the
+ * same arithmetic as dr_transform_homogeneous, one launch. */
+int dr_transform_homogeneous_multi(const float* matrices, int32_t views,
+                                   const float* vertices, int32_t dims,
+                                   int64_t num_vertices, float* out,
+                                   dr_stream_t stream);
+
 /* d_vertices[v, j] = sum_i M[i, j] d_clip[v, i], j < dims (overwrites). */
 int dr_transform_homogeneous_bwd(const float* matrix, const float* d_clip,
                                  int32_t dims, int64_t num_vertices,
