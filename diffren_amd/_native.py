@@ -18,9 +18,13 @@ _c_ptr = ctypes.c_void_p
 _c_size = ctypes.c_size_t
 
 
+RASTER_SPARSE = 1
+
+
 class RasterConfig(ctypes.Structure):
   _fields_ = [("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32),
-              ("bin_capacity", ctypes.c_int64)]
+              ("bin_capacity", ctypes.c_int64), ("flags", ctypes.c_int32),
+              ("reserved", ctypes.c_int32)]
 
 
 class RasterJob(ctypes.Structure):

@@ -53,14 +53,14 @@ def time_launch(fn, reps):
 
 
 def tune_raster(table_host, num_jobs, num_layers, culling, device,
-                job_sizes, reps=5, cache_path=None, candidates=None):
+                job_sizes, reps=5, cache_path=None, candidates=None, flags=0):
   """Returns (tile_w, tile_h, bin_capacity, timings_ms).
 
   table_host: ctypes array of dr_raster_job with the device pointers filled;
   job_sizes: list of (width, height, num_triangles) for the signature.
   """
   _load(cache_path)
-  key = signature(job_sizes, num_layers, culling)
+  key = signature(job_sizes, num_layers, culling) + f"/{flags}"
   if key in _cache:
     e = _cache[key]
     return e["tile"][0], e["tile"][1], e["capacity"], e["timings"]
@@ -70,7 +70,7 @@ def tune_raster(table_host, num_jobs, num_layers, culling, device,
   for tw, th in (candidates or CANDIDATE_TILES):
     if num_layers * tw * th * 8 > 65536:
       continue
-    cfg = _native.RasterConfig(tw, th, 0)
+    cfg = _native.RasterConfig(tw, th, 0, flags, 0)
     tt, tiles = ctypes.c_int64(), ctypes.c_int64()
     _native.check(lib.dr_raster_jobs_prepare(table_host, num_jobs,
                                              ctypes.byref(cfg),

@@ -102,7 +102,10 @@ class BatchedSplatRenderer:
     lib = _native.lib()
     if tile == "auto":
       tile, bin_capacity = self._autotune()
-    self.raster_config = rasterize._config(tile, bin_capacity)
+    # The fused passes read ids first, so background z / barycentrics are
+    # never written (DR_RASTER_SPARSE).
+    self.raster_config = rasterize._config(tile, bin_capacity,
+                                           _native.RASTER_SPARSE)
     n = len(self.jobs)
     table = self._raster_table()
     tt, tiles = ctypes.c_int64(), ctypes.c_int64()
@@ -141,7 +144,7 @@ class BatchedSplatRenderer:
              for mi, _, w, h in self.jobs]
     tw, th, cap, timings = autotune.tune_raster(
         self._raster_table(), len(self.jobs), self.L, 1, self.device, sizes,
-        cache_path=self.autotune_cache)
+        cache_path=self.autotune_cache, flags=_native.RASTER_SPARSE)
     self.autotune_timings = timings
     return (tw, th), cap
 

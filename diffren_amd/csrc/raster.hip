@@ -629,7 +629,7 @@ template <int TW, int TH>
 __global__ __launch_bounds__(kBlock) void raster_tile_kernel(
     Jobs J, int L, int cull, const uint32_t* __restrict__ counts,
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ bins,
-    uint64_t capacity) {
+    uint64_t capacity, bool sparse) {
   constexpr int TP = TW * TH;
   extern __shared__ uint64_t keys[];  // [L][TP]
   __shared__ TriRect queue[kBlock];
@@ -757,6 +757,7 @@ __global__ __launch_bounds__(kBlock) void raster_tile_kernel(
         shade_pixel(ix, iy, sx2, sy2, r, v, f);
       }
       job.triangle_ids[o] = id;
+      if (sparse && id < 0) continue;
       job.z_buffer[o] = f.z;
       if (job.barycentrics) {
         job.barycentrics[3 * o + 0] = f.b0;
@@ -824,7 +825,7 @@ uint64_t capacity_of(const dr_raster_config& c, int64_t total_tris,
 
 template <int TW, int TH>
 int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
-               int cull, const Workspace& w, uint64_t capacity,
+               int cull, const Workspace& w, uint64_t capacity, int flags,
                hipStream_t s) {
   const int64_t clear_n = total_tiles;
   clear_kernel<<<dim3(unsigned((clear_n + kBlock - 1) / kBlock)), kBlock, 0,
@@ -855,7 +856,8 @@ int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
   }
   const size_t lds = sizeof(uint64_t) * L * TW * TH;
   raster_tile_kernel<TW, TH><<<unsigned(total_tiles), kBlock, lds, s>>>(
-      J, L, cull, w.counts, w.offsets, w.bins, capacity);
+      J, L, cull, w.counts, w.offsets, w.bins, capacity,
+      (flags & DR_RASTER_SPARSE) != 0);
   DR_CHECK_LAUNCH("raster tiles");
   return kOk;
 }
@@ -894,7 +896,7 @@ int dispatch(const dr_raster_job* table, const dr_raster_job* single,
 #define DR_TILE_CASE(A, B)                                              \
   if (c.tile_w == A && c.tile_h == B)                                   \
     return launch_all<A, B>(J, total_tris, total_tiles, L, cull, w,     \
-                            capacity, s);
+                            capacity, c.flags, s);
   DR_TILE_CASE(8, 8)
   DR_TILE_CASE(16, 16)
   DR_TILE_CASE(32, 8)
@@ -906,7 +908,7 @@ int dispatch(const dr_raster_job* table, const dr_raster_job* single,
 }
 
 dr_raster_config resolve_config(const dr_raster_config* c) {
-  dr_raster_config r{32, 8, 0};
+  dr_raster_config r{32, 8, 0, 0, 0};
   if (c) r = *c;
   return r;
 }

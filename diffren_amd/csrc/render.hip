@@ -118,6 +118,13 @@ __device__ __forceinline__ float viewport_z(float z) {
   return ((z + 1.0f) * 1.0f) * 0.5f;
 }
 
+// Viewport z of pixel-layer q; background is the clear depth's, 1.  The z
+// buffer is read only under a triangle (the rasterizer's sparse mode leaves
+// background z unwritten).
+__device__ __forceinline__ float layer_z(const dr_splat_job& J, int64_t q) {
+  return J.triangle_ids[q] >= 0 ? viewport_z(J.z_buffer[q]) : 1.0f;
+}
+
 __device__ __forceinline__ int surface_index(const float* zS, const float* zT,
                                              int L) {
   float bc = fabsf(zS[0] - zT[0]);
@@ -176,32 +183,73 @@ __global__ __launch_bounds__(kBX* kBY) void target_kernel(
   const int64_t P = int64_t(W) * H;
   const int tid = threadIdx.y * kBX + threadIdx.x;
 
-  for (int i = tid; i < L * R; i += kBX * kBY) {
-    const int l = i / R, r = i % R;
-    const int px = x0 - 1 + r % kRX, py = y0 - 1 + r / kRX;
-    float x = __int_as_float(0x7fc00000), y = 0.0f, z = 1.0f;
-    float col[C];
+  // Each thread stages kChunk items at a time, issuing the loads of all of
+  // them stage by stage (ids; then triangles, barycentrics, z; then vertex
+  // gathers) so that kChunk dependent load chains are in flight at once.
+  constexpr int kChunk = 4;
+  for (int base = tid; base < L * R; base += kChunk * kBX * kBY) {
+    int tri[kChunk];
+    int64_t qs[kChunk];
 #pragma unroll
-    for (int c = 0; c < C; ++c) col[c] = 0.0f;
-    if (px >= 0 && px < W && py >= 0 && py < H) {
-      const int64_t q = l * P + int64_t(py) * W + px;
-      z = viewport_z(J.z_buffer[q]);
-      Corner k;
-      if (corners(J, q, k)) {
-        float cx, cy, cw;
-        splat_position(J, k, x, y, cx, cy, cw);
-        const float* a0 = J.colors + int64_t(k.v[0]) * C;
-        const float* a1 = J.colors + int64_t(k.v[1]) * C;
-        const float* a2 = J.colors + int64_t(k.v[2]) * C;
-#pragma unroll
-        for (int c = 0; c < C; ++c) col[c] = interp3(k, a0[c], a1[c], a2[c]);
+    for (int j = 0; j < kChunk; ++j) {
+      const int i = base + j * kBX * kBY;
+      tri[j] = -1;
+      qs[j] = 0;
+      if (i < L * R) {
+        const int l = i / R, r = i % R;
+        const int px = x0 - 1 + r % kRX, py = y0 - 1 + r / kRX;
+        if (px >= 0 && px < W && py >= 0 && py < H) {
+          qs[j] = l * P + int64_t(py) * W + px;
+          tri[j] = J.triangle_ids[qs[j]];
+        }
       }
     }
-    sx[i] = x;
-    sy[i] = y;
-    sz[i] = z;
+    Corner k[kChunk];
+    float zb[kChunk];
 #pragma unroll
-    for (int c = 0; c < C; ++c) sc[i * C + c] = col[c];
+    for (int j = 0; j < kChunk; ++j) {
+      const int t = tri[j];
+      zb[j] = 1.0f;
+      if (t >= 0 && t < J.num_triangles) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          k[j].v[e] = J.triangles[3 * int64_t(t) + e];
+          k[j].b[e] = J.barycentrics[3 * qs[j] + e];
+        }
+        zb[j] = J.z_buffer[qs[j]];
+      } else {
+        tri[j] = -1;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      const int i = base + j * kBX * kBY;
+      if (i >= L * R) break;
+      float x = __int_as_float(0x7fc00000), y = 0.0f, z = 1.0f;
+      float col[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) col[c] = 0.0f;
+      bool ok = tri[j] >= 0;
+#pragma unroll
+      for (int e = 0; e < 3; ++e)
+        ok = ok && unsigned(k[j].v[e]) < unsigned(J.num_vertices);
+      if (ok) {
+        z = viewport_z(zb[j]);
+        float cx, cy, cw;
+        splat_position(J, k[j], x, y, cx, cy, cw);
+        const float* a0 = J.colors + int64_t(k[j].v[0]) * C;
+        const float* a1 = J.colors + int64_t(k[j].v[1]) * C;
+        const float* a2 = J.colors + int64_t(k[j].v[2]) * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+          col[c] = interp3(k[j], a0[c], a1[c], a2[c]);
+      }
+      sx[i] = x;
+      sy[i] = y;
+      sz[i] = z;
+#pragma unroll
+      for (int c = 0; c < C; ++c) sc[i * C + c] = col[c];
+    }
   }
   __syncthreads();
 
@@ -373,7 +421,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   float zS[kMaxLayers], zT[kMaxLayers];
   int surf[9];
   if (any) {
-    for (int l = 0; l < L; ++l) zS[l] = viewport_z(J.z_buffer[l * P + p]);
+    for (int l = 0; l < L; ++l) zS[l] = layer_z(J, l * P + p);
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
       const int u = t / 3 - 1, v = t % 3 - 1;
@@ -381,7 +429,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
       surf[t] = 0;
       if (tx < 0 || tx >= W || ty < 0 || ty >= H) continue;
       const int64_t pt = int64_t(ty) * W + tx;
-      for (int l = 0; l < L; ++l) zT[l] = viewport_z(J.z_buffer[l * P + pt]);
+      for (int l = 0; l < L; ++l) zT[l] = layer_z(J, l * P + pt);
       surf[t] = surface_index(zS, zT, L);
     }
   }

@@ -28,9 +28,10 @@ DEFAULT_TILE = (32, 8)
 MAX_LAYERS = 8
 
 
-def _config(tile=None, bin_capacity=0):
+def _config(tile=None, bin_capacity=0, flags=0):
   tw, th = tile or DEFAULT_TILE
-  return _native.RasterConfig(int(tw), int(th), int(bin_capacity))
+  return _native.RasterConfig(int(tw), int(th), int(bin_capacity),
+                              int(flags), 0)
 
 
 def _check_inputs(vertices, triangles, image_width, image_height, num_layers,

@@ -79,7 +79,15 @@ typedef struct dr_raster_config {
   int32_t tile_w;
   int32_t tile_h;
   int64_t bin_capacity;
+  int32_t flags;     /* DR_RASTER_* bits */
+  int32_t reserved;  /* 0 */
 } dr_raster_config;
+
+/* z and barycentrics are written only where a triangle was stored
+ * (triangle_ids >= 0); background entries keep their previous contents and
+ * mean z 1, barycentrics 0.  For consumers that read ids first (the fused
+ * render), it saves the background writes of every layer. */
+enum { DR_RASTER_SPARSE = 1 };
 
 /* Host: fills triangle_begin / tile_begin of jobs_host[0..num_jobs) and
  * returns the totals.  Returns DR_OK or DR_INVALID_ARGUMENT. */
