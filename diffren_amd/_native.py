@@ -107,6 +107,7 @@ SIGNATURES = {
                              ctypes.c_float, ctypes.c_float, ctypes.c_float,
                              ctypes.c_float, _c_ptr]),
     "dr_splat_jobs_prepare": (_c_int, [_c_ptr, _c_int, _c_int, _c_ptr]),
+    "dr_render_scratch_floats": (_c_int, [_c_int]),
     "dr_render_record_size": (_c_int, [_c_int]),
     "dr_render_splat_workspace_size": (_c_size, [_c_i64]),
     "dr_render_splat_forward": (_c_int, [_c_ptr, _c_int, _c_i64, _c_int,

@@ -85,8 +85,9 @@ class BatchedSplatRenderer:
                                       device=dev))
       self.targets.append(torch.zeros((h, w, C), dtype=torch.float32,
                                       device=dev))
-      self.scratch.append(torch.empty((h, w, 3, C + 1), dtype=torch.float32,
-                                      device=dev))
+      self.scratch.append(torch.empty(
+          (h, w, _native.lib().dr_render_scratch_floats(C)),
+          dtype=torch.float32, device=dev))
       npix += w * h * C
     self.loss_scale = 1.0 / npix if loss_scale is None else float(loss_scale)
     self.cameras = [j[1].to(torch.float32).contiguous() for j in jobs]

@@ -649,12 +649,33 @@ __global__ __launch_bounds__(kBlock) void raster_tile_kernel(
   const float sx2 = 2.0f / static_cast<float>(W);
   const float sy2 = 2.0f / static_cast<float>(H);
 
-  for (int i = threadIdx.x; i < L * TP; i += kBlock) keys[i] = kEmpty;
-  for (int i = threadIdx.x; i < (TP + 31) / 32; i += kBlock) ties[i] = 0u;
   const uint32_t cnt = counts[flat_tile];
   const uint64_t off = offsets[flat_tile];
   const bool overflow = off + cnt > capacity;
   const int64_t items = overflow ? int64_t(job.num_triangles) : int64_t(cnt);
+  const int64_t P = int64_t(W) * H;
+  if (items == 0) {
+    // Empty tile (most of a frame): background everywhere, no LDS.
+    for (int p = threadIdx.x; p < TP; p += kBlock) {
+      const int ix = x0 + p % TW, iy = y0 + p / TW;
+      if (ix >= x1 || iy >= y1) continue;
+      const int64_t pix = int64_t(iy) * W + ix;
+      for (int l = 0; l < L; ++l) {
+        const int64_t o = pix + l * P;
+        job.triangle_ids[o] = -1;
+        if (sparse) continue;
+        job.z_buffer[o] = 1.0f;
+        if (job.barycentrics) {
+          job.barycentrics[3 * o + 0] = 0.0f;
+          job.barycentrics[3 * o + 1] = 0.0f;
+          job.barycentrics[3 * o + 2] = 0.0f;
+        }
+      }
+    }
+    return;
+  }
+  for (int i = threadIdx.x; i < L * TP; i += kBlock) keys[i] = kEmpty;
+  for (int i = threadIdx.x; i < (TP + 31) / 32; i += kBlock) ties[i] = 0u;
 
   for (int64_t base = 0; base < items; base += kBlock) {
     if (threadIdx.x == 0) queue_n = 0;
@@ -737,7 +758,6 @@ __global__ __launch_bounds__(kBlock) void raster_tile_kernel(
   }
 
   // Resolve: recompute the winning fragments exactly and write the outputs.
-  const int64_t P = int64_t(W) * H;
   for (int p = threadIdx.x; p < TP; p += kBlock) {
     const int ix = x0 + p % TW, iy = y0 + p / TW;
     if (ix >= x1 || iy >= y1) continue;

@@ -46,6 +46,16 @@ constexpr int kRX = kBX + 2, kRY = kBY + 2, kRegion = kRX * kRY;
 constexpr int kMaxLayers = 8;
 constexpr int kMaxChannels = 16;
 
+// Per-pixel gradient scratch written by the target pass for the source pass:
+// 3 classes x kStride floats (d acc colour [C], d acc weight, padding to a
+// float4 multiple), then 16 bytes holding the pixel's 9 surface indices
+// (one signed byte per tap, tap order t = 3 (u + 1) + (v + 1)).
+template <int C>
+struct Scratch {
+  static constexpr int kStride = ((C + 1) + 3) & ~3;
+  static constexpr int kRecord = 3 * kStride + 4;  // floats
+};
+
 __device__ __forceinline__ int find_job(const dr_splat_job* jobs, int n,
                                         int64_t tile) {
   int lo = 0, hi = n - 1;
@@ -336,18 +346,32 @@ __global__ __launch_bounds__(kBX* kBY) void target_kernel(
       }
       dn[0][C - 1] += da0;
       dn[1][C - 1] += da1;
-      float* gp = J.grad_scratch + p * 3 * (C + 1);
+      constexpr int GS = Scratch<C>::kStride;
+      float* gp = J.grad_scratch + p * Scratch<C>::kRecord;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const float ds = acc[k][C] > 1.0f ? -(s[k] * s[k]) : 0.0f;
+        float rec[GS];
         float dw = 0.0f;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          gp[k * (C + 1) + c] = dn[k][c] * s[k];
+          rec[c] = dn[k][c] * s[k];
           dw += dn[k][c] * acc[k][c];
         }
-        gp[k * (C + 1) + C] = dw * ds;
+        rec[C] = dw * ds;
+#pragma unroll
+        for (int c = C + 1; c < GS; ++c) rec[c] = 0.0f;
+#pragma unroll
+        for (int c = 0; c < GS; c += 4)
+          *reinterpret_cast<float4*>(gp + k * GS + c) =
+              make_float4(rec[c], rec[c + 1], rec[c + 2], rec[c + 3]);
       }
+      uint32_t sw[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+      for (int t = 0; t < 9; ++t)
+        sw[t >> 2] |= (uint32_t(surf[t]) & 0xffu) << (8 * (t & 3));
+      *reinterpret_cast<uint4*>(gp + 3 * GS) =
+          make_uint4(sw[0], sw[1], sw[2], sw[3]);
     }
   }
   if (grad_mode == 1) {
@@ -387,11 +411,19 @@ struct Record {
   static constexpr int kSize = (4 + C) <= 8 ? 8 : ((4 + C) <= 16 ? 16 : 32);
 };
 
-// Source pass.  Float atomics are the cost of this pass; they run ~7x faster
-// when the lanes of one atomic instruction cover contiguous records than
-// when each lane hits its own scattered address, so each wave stages its
-// fragments' three vertex records in LDS and flushes them with kSize
-// consecutive lanes per record.
+// Source pass.  Float atomics to the vertex gradient records are the cost of
+// this pass (they run at a fixed rate of grouped operations), so it issues as
+// few as it can:
+//  * each wave covers a 16x4 patch of pixels, whose fragments reference each
+//    mesh vertex ~4 times on average (neighbouring pixels' triangles share
+//    vertices);
+//  * the wave stages its fragments' three vertex records in LDS, finds
+//    repeated vertices with a small LDS hash table, adds each repeat into
+//    the first occurrence's record, and flushes only first occurrences;
+//  * the flush puts R consecutive lanes on one record, because grouped
+//    atomics run ~7x faster than one scattered address per lane.
+constexpr int kHashSlots = 256;   // > 3 x 64 entries per wave and layer
+
 template <int C>
 __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     const dr_splat_job* __restrict__ jobs, int num_jobs, int L, float coef) {
@@ -399,17 +431,22 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   constexpr int kPerWave = 3 * 64;             // entries per wave and layer
   __shared__ float stage_val[kBX * kBY / 64][kPerWave][R];
   __shared__ int stage_vtx[kBX * kBY / 64][kPerWave];
+  __shared__ int hash_key[kBX * kBY / 64][kHashSlots];
+  __shared__ int hash_own[kBX * kBY / 64][kHashSlots];
   const int64_t tile = blockIdx.x;
   const dr_splat_job& J = jobs[find_job(jobs, num_jobs, tile)];
   const int W = J.width, H = J.height;
   const int ntx = (W + kBX - 1) / kBX;
   const int local = int(tile - J.tile_begin);
-  const int x = (local % ntx) * kBX + threadIdx.x;
-  const int y = (local / ntx) * kBY + threadIdx.y;
   const int lane = __lane_id();
   const int wave = (threadIdx.y * kBX + threadIdx.x) >> 6;
+  // Wave w covers columns 16w .. 16w+15 of the 64x4 tile, all 4 rows.
+  const int x = (local % ntx) * kBX + 16 * wave + (lane & 15);
+  const int y = (local / ntx) * kBY + (lane >> 4);
   float(*val)[R] = stage_val[wave];
   int* vtx = stage_vtx[wave];
+  int* hkey = hash_key[wave];
+  int* hown = hash_own[wave];
   const bool inside = x < W && y < H;
   const int64_t P = int64_t(W) * H;
   const int64_t p = int64_t(y) * W + x;
@@ -418,19 +455,20 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   if (inside)
     for (int l = 0; l < L; ++l) any |= J.triangle_ids[l * P + p] >= 0;
 
-  float zS[kMaxLayers], zT[kMaxLayers];
+  // Surface indices of this source with its 9 targets, as the target pass
+  // computed them: byte t of each target's scratch record.
+  constexpr int GS = Scratch<C>::kStride, GR = Scratch<C>::kRecord;
   int surf[9];
   if (any) {
-    for (int l = 0; l < L; ++l) zS[l] = layer_z(J, l * P + p);
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
       const int u = t / 3 - 1, v = t % 3 - 1;
       const int tx = x + u, ty = y + v;
       surf[t] = 0;
       if (tx < 0 || tx >= W || ty < 0 || ty >= H) continue;
-      const int64_t pt = int64_t(ty) * W + tx;
-      for (int l = 0; l < L; ++l) zT[l] = layer_z(J, l * P + pt);
-      surf[t] = surface_index(zS, zT, L);
+      const int8_t* sb = reinterpret_cast<const int8_t*>(
+          J.grad_scratch + (int64_t(ty) * W + tx) * GR + 3 * GS);
+      surf[t] = sb[t];
     }
   }
   // Every lane of the wave runs every layer iteration (ballots below).
@@ -462,12 +500,18 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         const float w = tap_weight(coef, sx, sy, u, v, dx, dy);
         const int cls = layer_class(surf[t], l);
         const float* gp =
-            J.grad_scratch + ((int64_t(ty) * W + tx) * 3 + cls) * (C + 1);
-        float dw = gp[C];
+            J.grad_scratch + (int64_t(ty) * W + tx) * GR + cls * GS;
+        float gv[GS];
+#pragma unroll
+        for (int c = 0; c < GS; c += 4) {
+          const float4 q4 = *reinterpret_cast<const float4*>(gp + c);
+          gv[c] = q4.x; gv[c + 1] = q4.y; gv[c + 2] = q4.z; gv[c + 3] = q4.w;
+        }
+        float dw = gv[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          dcol[c] += w * gp[c];
-          dw += gp[c] * col[c];
+          dcol[c] += w * gv[c];
+          dw += gv[c] * col[c];
         }
         const float dw4 = dw * 4.0f * w;
         gx += dw4 * dx;
@@ -501,16 +545,53 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         for (int c = 4 + C; c < R; ++c) rec[i][c] = 0.0f;
       }
     }
-    // Wave-local append of 3 entries per fragment, then a grouped flush.
+    // Wave-local append of 3 entries per fragment, dedupe, grouped flush.
     const uint64_t act = __ballot(frag);
     if (act == 0) continue;
+    for (int i = lane; i < kHashSlots; i += 64) hkey[i] = -1;
+    int base = 0;
+    int slot[3] = {0, 0, 0};
+    bool first[3] = {false, false, false};
     if (frag) {
-      const int base = 3 * __popcll(act & ((uint64_t(1) << lane) - 1));
+      base = 3 * __popcll(act & ((uint64_t(1) << lane) - 1));
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         vtx[base + i] = k.v[i];
 #pragma unroll
         for (int c = 0; c < R; ++c) val[base + i][c] = rec[i][c];
+      }
+    }
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+    if (frag) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        // Linear probing; the table has more slots than a wave has entries,
+        // so every probe sequence ends at a free slot or a match.
+        uint32_t h = (uint32_t(k.v[i]) * 2654435761u) >> 24;
+        for (int probe = 0; probe < kHashSlots; ++probe) {
+          const int old = atomicCAS(&hkey[h], -1, k.v[i]);
+          if (old == -1) { first[i] = true; break; }
+          if (old == k.v[i]) break;
+          h = (h + 1) & (kHashSlots - 1);
+        }
+        slot[i] = int(h);
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        if (first[i]) hown[slot[i]] = base + i;
+    }
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
+    if (frag) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        if (first[i]) continue;
+        const int o = hown[slot[i]];
+#pragma unroll
+        for (int c = 0; c < R; ++c)
+          if (rec[i][c] != 0.0f) atomicAdd(&val[o][c], rec[i][c]);
+        vtx[base + i] = -1;
       }
     }
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
@@ -521,8 +602,10 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
       for (int e0 = 0; e0 < n; e0 += 64 / R) {
         const int e = e0 + lane / R, c = lane % R;
         if (e < n) {
+          const int vv = vtx[e];
           const float g = val[e][c];
-          if (g != 0.0f) unsafeAtomicAdd(grads + int64_t(vtx[e]) * R + c, g);
+          if (vv >= 0 && g != 0.0f)
+            unsafeAtomicAdd(grads + int64_t(vv) * R + c, g);
         }
       }
     }
@@ -625,6 +708,11 @@ int dr_splat_jobs_prepare(dr_splat_job* jobs, int32_t n, int32_t channels,
   }
   if (total_tiles) *total_tiles = tiles;
   return DR_OK;
+}
+
+int32_t dr_render_scratch_floats(int32_t channels) {
+  if (channels < 1 || channels > dr::kMaxChannels) return 0;
+  return 3 * (((channels + 1) + 3) & ~3) + 4;
 }
 
 int32_t dr_render_record_size(int32_t channels) {

@@ -241,7 +241,8 @@ typedef struct dr_splat_job {
   float* image;                 /* [H, W, C] output, or NULL */
   const float* target;          /* [H, W, C] L2 target (grad_mode 1) */
   const float* d_image;         /* [H, W, C] upstream gradient (grad_mode 2) */
-  float* grad_scratch;          /* [H, W, 3, C + 1] per-job workspace */
+  float* grad_scratch;          /* [H, W, dr_render_scratch_floats(C)]
+                                   per-job workspace, 16-byte aligned */
   float* d_records;             /* [V, R] accumulated gradient records, or
                                    NULL; R = dr_render_record_size(C).
                                    cols 0-2: d vertex = M^T d clip when
@@ -258,6 +259,9 @@ typedef struct dr_splat_job {
 /* Host: fills tile_begin of jobs[0..n) and returns the total tile count. */
 int dr_splat_jobs_prepare(dr_splat_job* jobs_host, int32_t num_jobs,
                           int32_t channels, int64_t* total_tiles);
+
+/* Floats per pixel of a job's grad_scratch (0 if unsupported). */
+int32_t dr_render_scratch_floats(int32_t channels);
 
 /* Floats per vertex gradient record for `channels` colour channels (8, 16
  * or 32; 0 if unsupported). */
