@@ -46,6 +46,8 @@ def parse():
   p.add_argument("--no-cpu-baseline", action="store_true")
   p.add_argument("--no-grad-check", action="store_true")
   p.add_argument("--no-graph", action="store_true")
+  p.add_argument("--streams", type=int, default=2,
+                 help="concurrent job groups in the fused step")
   return p.parse_args()
 
 
@@ -199,7 +201,8 @@ def main():
   w["targets"] = [img.clone() for img in tr.images]
   del tr
 
-  r = batch.BatchedSplatRenderer([w["mesh"]], jobs, L, tile="auto")
+  r = batch.BatchedSplatRenderer([w["mesh"]], jobs, L, tile="auto",
+                                 streams=args.streams)
   log(f"autotuned raster tile {r.raster_config.tile_w}x"
       f"{r.raster_config.tile_h}, bin capacity {r.raster_config.bin_capacity}"
       f", timings {r.autotune_timings}")
@@ -299,7 +302,7 @@ def main():
       "data": "synthetic (seeded Perlin-displaced UV sphere, seeded cameras)",
       "config": {"workload": "1M-triangle sphere, 1024x1024, 3 layers, 8 "
                              "views, RGBA, fwd + L2 loss + bwd, batched "
-                             "fused path" +
+                             f"fused path on {args.streams} stream(s)" +
                              (" in a HIP graph" if use_graph else ""),
                  "triangles": T, "vertices": V, "resolution": R, "layers": L,
                  "views": B, "channels": C, "parallelism": "single GPU"},

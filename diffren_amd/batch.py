@@ -3,12 +3,15 @@
 `BatchedSplatRenderer` renders many (mesh, camera, resolution) jobs -- of
 uneven triangle counts and image sizes -- through one set of launches per
 stage:
-  transform (per job) -> batched tile rasterizer (one launch per stage for all
-  jobs) -> fused target pass (interpolate + splat + over + L2 gradient) ->
-  fused source pass (splat VJP + interpolate VJP + transform VJP, scattering
-  into d colours and d vertices).
-All buffers are allocated once at construction, so a step allocates nothing
-and can be captured in a HIP graph (`capture()` / `replay()`).
+  transform (one launch per mesh, all its views) -> batched tile rasterizer
+  (one launch per stage for all jobs of a group) -> fused target pass
+  (interpolate + splat + over + L2 gradient) -> fused source pass (splat VJP
+  + interpolate VJP + transform VJP, scattering into d colours and
+  d vertices).
+Jobs can be split into `streams` groups that run concurrently on their own
+streams (the tail of one group's kernels overlaps the other's work).  All
+buffers are allocated once at construction, so a step allocates nothing and
+is captured in a HIP graph (`capture()` / `replay()`).
 
 This is the batched counterpart of calling render.render_triangles(...,
 shading_function=lambda a: a["colors"], compositing_mode=SPLAT_OVER) once per
@@ -23,6 +26,13 @@ from diffren_amd import _native
 from diffren_amd import rasterize
 
 
+class _Group:
+  """Jobs of one stream: their raster and splat job tables and workspaces."""
+
+  def __init__(self, indices):
+    self.indices = indices
+
+
 class BatchedSplatRenderer:
   """Preplanned fused forward+backward over a fixed batch of render jobs.
 
@@ -30,10 +40,19 @@ class BatchedSplatRenderer:
            "triangles" [T,3] i32, "colors" [V,C] f32; jobs reference a mesh
            by index, so several views of one mesh share its gradient buffers.
   jobs:    list of (mesh_index, camera [4,4] device f32, width, height).
+           Cameras are copied at construction.
+  tile:    raster tile (w, h), None for the default or "auto" for the
+           autotuner (diffren_amd.autotune).
+  streams: number of job groups run concurrently (1 or 2 are sensible).
+
+  After step(): images[i], loss (sum over jobs of loss_scale * |img - tgt|^2),
+  d_vertices[m] [V,3] and d_colors[m] [V,C] (views of records[m]).  ids[i]
+  holds triangle ids; z[i] / barys[i] are defined under triangles only.
   """
 
   def __init__(self, meshes, jobs, num_layers, tile=None, bin_capacity=0,
-               loss_scale=None, autotune_cache=None):
+               loss_scale=None, autotune_cache=None, streams=1):
+    lib = _native.lib()
     self.meshes = meshes
     self.jobs = jobs
     self.L = int(num_layers)
@@ -46,7 +65,7 @@ class BatchedSplatRenderer:
     self.C = C
     # Per-mesh gradient records [V, R]: d position in cols 0..2, d colour in
     # cols 4..4+C-1 (one contiguous record per vertex for grouped atomics).
-    R = _native.lib().dr_render_record_size(C)
+    R = lib.dr_render_record_size(C)
     if R <= 0:
       raise ValueError(f"unsupported channel count {C}")
     self.records = [torch.zeros((m["vertices"].shape[0], R),
@@ -55,7 +74,7 @@ class BatchedSplatRenderer:
     self.d_vertices = [r[:, 0:3] for r in self.records]
     self.d_colors = [r[:, 4:4 + C] for r in self.records]
     self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
-    self.clip, self.ids, self.z, self.barys = [], [], [], []
+    self.ids, self.z, self.barys = [], [], []
     self.images, self.targets, self.scratch = [], [], []
     # Clip-space vertices: one [views, V, 4] block per mesh, one transform
     # launch per mesh.
@@ -69,12 +88,11 @@ class BatchedSplatRenderer:
                                          dtype=torch.float32, device=dev)
       self.camera_blocks[mi] = torch.stack(
           [jobs[i][1].to(torch.float32).reshape(4, 4) for i in views])
-    slot = {}
-    for i, (mi, _, _, _) in enumerate(jobs):
-      slot[i] = self.mesh_views[mi].index(i)
+    self.clip = [self.clip_blocks[mi][self.mesh_views[mi].index(i)]
+                 for i, (mi, _, _, _) in enumerate(jobs)]
+    sf = lib.dr_render_scratch_floats(C)
     npix = 0
-    for i, (mi, cam, w, h) in enumerate(jobs):
-      self.clip.append(self.clip_blocks[mi][slot[i]])
+    for mi, cam, w, h in jobs:
       self.ids.append(torch.empty((self.L, h, w), dtype=torch.int32,
                                   device=dev))
       self.z.append(torch.empty((self.L, h, w), dtype=torch.float32,
@@ -85,56 +103,56 @@ class BatchedSplatRenderer:
                                       device=dev))
       self.targets.append(torch.zeros((h, w, C), dtype=torch.float32,
                                       device=dev))
-      self.scratch.append(torch.empty(
-          (h, w, _native.lib().dr_render_scratch_floats(C)),
-          dtype=torch.float32, device=dev))
+      self.scratch.append(torch.empty((h, w, sf), dtype=torch.float32,
+                                      device=dev))
       npix += w * h * C
     self.loss_scale = 1.0 / npix if loss_scale is None else float(loss_scale)
     self.cameras = [j[1].to(torch.float32).contiguous() for j in jobs]
     self.autotune_cache = autotune_cache
     self.autotune_timings = None
-    self._plan_raster(tile, bin_capacity)
-    self._plan_splat()
-    self.graph = None
-
-  # -- planning --------------------------------------------------------------
-
-  def _plan_raster(self, tile, bin_capacity):
-    lib = _native.lib()
+    self.groups = self._split(max(1, int(streams)))
+    self.streams = [torch.cuda.Stream(device=dev) for _ in self.groups] \
+        if len(self.groups) > 1 else []
     if tile == "auto":
       tile, bin_capacity = self._autotune()
     # The fused passes read ids first, so background z / barycentrics are
     # never written (DR_RASTER_SPARSE).
     self.raster_config = rasterize._config(tile, bin_capacity,
                                            _native.RASTER_SPARSE)
-    n = len(self.jobs)
-    table = self._raster_table()
-    tt, tiles = ctypes.c_int64(), ctypes.c_int64()
-    _native.check(lib.dr_raster_jobs_prepare(table, n,
-                                             ctypes.byref(self.raster_config),
-                                             ctypes.byref(tt),
-                                             ctypes.byref(tiles)),
-                  "dr_raster_jobs_prepare")
-    self.raster_table = _native.table_to_device(table, self.device)
-    self.total_triangles, self.raster_tiles = tt.value, tiles.value
-    nbytes = lib.dr_raster_workspace_size(tt.value, tiles.value,
-                                          ctypes.byref(self.raster_config))
-    self.raster_ws = _native.workspace(nbytes, self.device)
+    for g in self.groups:
+      self._plan_raster(g)
+      self._plan_splat(g)
+    self.graph = None
 
-  def _raster_table(self):
-    n = len(self.jobs)
-    table = (_native.RasterJob * n)()
-    for i, (mi, _, w, h) in enumerate(self.jobs):
+  # -- planning --------------------------------------------------------------
+
+  def _split(self, n):
+    """Splits jobs into n groups of balanced pixel counts (greedy)."""
+    n = min(n, len(self.jobs))
+    order = sorted(range(len(self.jobs)),
+                   key=lambda i: -self.jobs[i][2] * self.jobs[i][3])
+    load = [0] * n
+    parts = [[] for _ in range(n)]
+    for i in order:
+      k = load.index(min(load))
+      parts[k].append(i)
+      load[k] += self.jobs[i][2] * self.jobs[i][3]
+    return [_Group(sorted(p)) for p in parts if p]
+
+  def _raster_table(self, indices):
+    table = (_native.RasterJob * len(indices))()
+    for k, i in enumerate(indices):
+      mi, _, w, h = self.jobs[i]
       m = self.meshes[mi]
-      table[i].vertices = self.clip[i].data_ptr()
-      table[i].triangles = m["triangles"].data_ptr()
-      table[i].triangle_ids = self.ids[i].data_ptr()
-      table[i].z_buffer = self.z[i].data_ptr()
-      table[i].barycentrics = self.barys[i].data_ptr()
-      table[i].num_vertices = m["vertices"].shape[0]
-      table[i].num_triangles = m["triangles"].shape[0]
-      table[i].width = w
-      table[i].height = h
+      table[k].vertices = self.clip[i].data_ptr()
+      table[k].triangles = m["triangles"].data_ptr()
+      table[k].triangle_ids = self.ids[i].data_ptr()
+      table[k].z_buffer = self.z[i].data_ptr()
+      table[k].barycentrics = self.barys[i].data_ptr()
+      table[k].num_vertices = m["vertices"].shape[0]
+      table[k].num_triangles = m["triangles"].shape[0]
+      table[k].width = w
+      table[k].height = h
     return table
 
   def _autotune(self):
@@ -144,18 +162,35 @@ class BatchedSplatRenderer:
     sizes = [(w, h, self.meshes[mi]["triangles"].shape[0])
              for mi, _, w, h in self.jobs]
     tw, th, cap, timings = autotune.tune_raster(
-        self._raster_table(), len(self.jobs), self.L, 1, self.device, sizes,
-        cache_path=self.autotune_cache, flags=_native.RASTER_SPARSE)
+        self._raster_table(list(range(len(self.jobs)))), len(self.jobs),
+        self.L, 1, self.device, sizes, cache_path=self.autotune_cache,
+        flags=_native.RASTER_SPARSE)
     self.autotune_timings = timings
     return (tw, th), cap
 
-  def _plan_splat(self):
+  def _plan_raster(self, g):
     lib = _native.lib()
-    n = len(self.jobs)
-    table = (_native.SplatJob * n)()
-    for i, (mi, _, w, h) in enumerate(self.jobs):
+    table = self._raster_table(g.indices)
+    tt, tiles = ctypes.c_int64(), ctypes.c_int64()
+    _native.check(lib.dr_raster_jobs_prepare(table, len(g.indices),
+                                             ctypes.byref(self.raster_config),
+                                             ctypes.byref(tt),
+                                             ctypes.byref(tiles)),
+                  "dr_raster_jobs_prepare")
+    g.raster_table = _native.table_to_device(table, self.device)
+    g.total_triangles, g.raster_tiles = tt.value, tiles.value
+    g.raster_ws = _native.workspace(
+        lib.dr_raster_workspace_size(tt.value, tiles.value,
+                                     ctypes.byref(self.raster_config)),
+        self.device)
+
+  def _plan_splat(self, g):
+    lib = _native.lib()
+    table = (_native.SplatJob * len(g.indices))()
+    for k, i in enumerate(g.indices):
+      mi, _, w, h = self.jobs[i]
       m = self.meshes[mi]
-      j = table[i]
+      j = table[k]
       j.clip_vertices = self.clip[i].data_ptr()
       j.triangles = m["triangles"].data_ptr()
       j.colors = m["colors"].data_ptr()
@@ -173,12 +208,12 @@ class BatchedSplatRenderer:
       j.width = w
       j.height = h
     tiles = ctypes.c_int64()
-    _native.check(lib.dr_splat_jobs_prepare(table, n, self.C,
+    _native.check(lib.dr_splat_jobs_prepare(table, len(g.indices), self.C,
                                             ctypes.byref(tiles)),
                   "dr_splat_jobs_prepare")
-    self.splat_table = _native.table_to_device(table, self.device)
-    self.splat_tiles = tiles.value
-    self.splat_ws = _native.workspace(
+    g.splat_table = _native.table_to_device(table, self.device)
+    g.splat_tiles = tiles.value
+    g.splat_ws = _native.workspace(
         lib.dr_render_splat_workspace_size(tiles.value), self.device)
 
   # -- stages ----------------------------------------------------------------
@@ -194,31 +229,39 @@ class BatchedSplatRenderer:
               v.shape[1], v.shape[0], _native.ptr(self.clip_blocks[mi]), s),
           "dr_transform_homogeneous_multi")
 
-  def rasterize(self):
-    lib = _native.lib()
+  def _rasterize(self, g):
     _native.check(
-        lib.dr_rasterize_triangles_batched(
-            _native.ptr(self.raster_table), len(self.jobs),
-            self.total_triangles, self.raster_tiles, self.L, 1,
-            ctypes.byref(self.raster_config), _native.ptr(self.raster_ws),
-            self.raster_ws.numel(), _native.stream_handle()),
-        "dr_rasterize_triangles_batched")
+        _native.lib().dr_rasterize_triangles_batched(
+            _native.ptr(g.raster_table), len(g.indices), g.total_triangles,
+            g.raster_tiles, self.L, 1, ctypes.byref(self.raster_config),
+            _native.ptr(g.raster_ws), g.raster_ws.numel(),
+            _native.stream_handle()), "dr_rasterize_triangles_batched")
 
-  def splat_forward(self, grad_mode=1):
+  def _splat_forward(self, g, grad_mode):
     _native.check(
         _native.lib().dr_render_splat_forward(
-            _native.ptr(self.splat_table), len(self.jobs), self.splat_tiles,
-            self.L, self.C, grad_mode, self.loss_scale,
-            _native.ptr(self.loss), _native.ptr(self.splat_ws),
-            self.splat_ws.numel(), _native.stream_handle()),
-        "dr_render_splat_forward")
+            _native.ptr(g.splat_table), len(g.indices), g.splat_tiles, self.L,
+            self.C, grad_mode, self.loss_scale, _native.ptr(self.loss),
+            _native.ptr(g.splat_ws), g.splat_ws.numel(),
+            _native.stream_handle()), "dr_render_splat_forward")
 
-  def splat_backward(self):
+  def _splat_backward(self, g):
     _native.check(
         _native.lib().dr_render_splat_backward(
-            _native.ptr(self.splat_table), len(self.jobs), self.splat_tiles,
-            self.L, self.C, _native.stream_handle()),
-        "dr_render_splat_backward")
+            _native.ptr(g.splat_table), len(g.indices), g.splat_tiles, self.L,
+            self.C, _native.stream_handle()), "dr_render_splat_backward")
+
+  def rasterize(self):
+    for g in self.groups:
+      self._rasterize(g)
+
+  def splat_forward(self, grad_mode=1):
+    for g in self.groups:
+      self._splat_forward(g, grad_mode)
+
+  def splat_backward(self):
+    for g in self.groups:
+      self._splat_backward(g)
 
   def zero_grads(self):
     # Fill kernels, not memsets, so the step can be graph-captured.
@@ -226,19 +269,35 @@ class BatchedSplatRenderer:
       t.zero_()
     self.loss.zero_()
 
+  def _group_step(self, g, grad_mode):
+    self._rasterize(g)
+    self._splat_forward(g, grad_mode)
+    if grad_mode:
+      self._splat_backward(g)
+
+  def _run_groups(self, grad_mode):
+    if not self.streams:
+      self._group_step(self.groups[0], grad_mode)
+      return
+    cur = torch.cuda.current_stream()
+    for s in self.streams:
+      s.wait_stream(cur)
+    for g, s in zip(self.groups, self.streams):
+      with torch.cuda.stream(s):
+        self._group_step(g, grad_mode)
+    for s in self.streams:
+      cur.wait_stream(s)
+
   def step(self):
     """One forward + L2 loss + backward over all jobs (enqueue only)."""
     self.zero_grads()
     self.transform()
-    self.rasterize()
-    self.splat_forward(grad_mode=1)
-    self.splat_backward()
+    self._run_groups(1)
 
   def render(self):
     """Forward only: fills self.images."""
     self.transform()
-    self.rasterize()
-    self.splat_forward(grad_mode=0)
+    self._run_groups(0)
 
   # -- graphs ----------------------------------------------------------------
 
@@ -264,8 +323,14 @@ class BatchedSplatRenderer:
     self.graph.replay()
 
   def bin_stats(self):
-    need, over = ctypes.c_int64(), ctypes.c_int32()
-    _native.check(_native.lib().dr_raster_bin_stats(
-        _native.ptr(self.raster_ws), ctypes.byref(need), ctypes.byref(over),
-        _native.stream_handle()), "dr_raster_bin_stats")
-    return need.value, bool(over.value)
+    """(entries needed, overflowed) of the last raster launch of each group,
+    summed / or-ed."""
+    need, over = 0, False
+    for g in self.groups:
+      n, o = ctypes.c_int64(), ctypes.c_int32()
+      _native.check(_native.lib().dr_raster_bin_stats(
+          _native.ptr(g.raster_ws), ctypes.byref(n), ctypes.byref(o),
+          _native.stream_handle()), "dr_raster_bin_stats")
+      need += n.value
+      over = over or bool(o.value)
+    return need, over

@@ -400,7 +400,9 @@ __global__ __launch_bounds__(1024) void loss_reduce_kernel(
     if (int(threadIdx.x) < d) red[threadIdx.x] += red[threadIdx.x + d];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *loss += red[0];
+  // An atomic add, because job groups on concurrent streams share *loss;
+  // with two groups the result does not depend on their order.
+  if (threadIdx.x == 0) unsafeAtomicAdd(loss, red[0]);
 }
 
 // Gradient record of one vertex: [R] floats, R = record_size(C):
