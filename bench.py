@@ -48,6 +48,9 @@ def parse():
   p.add_argument("--no-graph", action="store_true")
   p.add_argument("--streams", type=int, default=2,
                  help="concurrent job groups in the fused step")
+  p.add_argument("--pmc-probe", action="store_true",
+                 help="run only the fused step + a calibration copy, for "
+                 "rocprofv3 --pmc passes")
   return p.parse_args()
 
 
@@ -86,6 +89,51 @@ def time_stage(fn, reps):
   end.record()
   torch.cuda.synchronize()
   return start.elapsed_time(end) / reps  # ms
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def measured_traffic(stage, groups):
+  """HBM bytes per launch of the stage's kernel from the committed
+  rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel
+  calibration recorded with them), or None when not measured."""
+  if not os.path.exists(TRAFFIC_FILE):
+    return None
+  with open(TRAFFIC_FILE) as f:
+    t = json.load(f)
+  e = t.get("kernels", {}).get(stage)
+  if not e:
+    return None
+  per_step = (e["fetch_kib"] * t["fetch_scale"] +
+              e["write_kib"] * t["write_scale"]) * 1024 * e["launches"]
+  return int(per_step / groups)
+
+
+def pmc_probe(args):
+  """Program for rocprofv3 --pmc passes: the fused step (eager) three times
+  with the default job groups, then a 256 MiB device copy whose FETCH_SIZE /
+  WRITE_SIZE calibrate the counters' units."""
+  import torch
+  from diffren_amd import batch
+  dev = torch.device("cuda:0")
+  w = build_workload(args, dev)
+  R, L = args.resolution, args.layers
+  jobs = [(0, w["cams_t"][i], R, R) for i in range(args.views)]
+  r = batch.BatchedSplatRenderer([w["mesh"]], jobs, L, tile=(16, 16),
+                                 streams=args.streams)
+  for t in r.targets:
+    t.uniform_()
+  for _ in range(3):
+    r.step()
+  torch.cuda.synchronize()
+  a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+  a.fill_(1.0)
+  b = torch.empty_like(a)
+  torch.cuda.synchronize()
+  b.copy_(a)
+  torch.cuda.synchronize()
+  print("pmc probe done", flush=True)
 
 
 def plain_step(w, args):
@@ -180,10 +228,12 @@ def main():
           "gradient all-reduce, which this build does not implement)",
           file=sys.stderr)
     sys.exit(2)
+  if args.pmc_probe:
+    pmc_probe(args)
+    return
   import numpy as np
   import torch
   from diffren_amd import batch
-  from diffren_amd import rasterize
 
   dev = torch.device("cuda:0")
   log("building workload")
@@ -242,26 +292,39 @@ def main():
   # and output counted once.
   P = R * R
   lp = L * P
-  bytes_target = B * (lp * (4 + 12 + 4) + T * 12 + V * 16 + V * 4 * C +
-                      P * 4 * C + P * 4 * C + P * 3 * (C + 1) * 4)
-  bytes_source = B * (lp * (4 + 12 + 4) + T * 12 + V * 16 + V * 4 * C +
-                      P * 3 * (C + 1) * 4 + 16 * 4) + V * 4 * (C + 3)
+  # Per view: the API-level inputs and outputs of each pass, once each:
+  #   target: ids, barycentrics, z (L*P*20), triangles, clip vertices,
+  #           colours, target image -> image, gradient scratch
+  #           (3 classes x (C + 1) floats + 9 surface-index bytes per pixel);
+  #   source: ids, barycentrics (L*P*16), gradient scratch, triangles, clip
+  #           vertices, colours, camera -> vertex gradients (3 + C floats
+  #           per vertex, written once per step).
+  scratch = 3 * (C + 1) * 4 + 9
+  bytes_target = B * (lp * 20 + T * 12 + V * 16 + V * 4 * C + P * 4 * C +
+                      P * 4 * C + P * scratch)
+  bytes_source = B * (lp * 16 + P * scratch + T * 12 + V * 16 + V * 4 * C +
+                      64) + V * 4 * (C + 3)
   bytes_raster = B * (T * 12 + V * 16 + lp * (4 + 4 + 12))
+  # The dominant single kernel (the raster stage is several kernels and is
+  # reported in stage_ms).  Stages launch once per job group.
+  ng = len(r.groups)
   per_stage = {
-      "splat_target": ("render_splat target pass (interpolate+splat+over+L2"
-                       " grad)", bytes_target),
-      "splat_source": ("render_splat source pass (splat/interpolate/"
-                       "transform VJP)", bytes_source),
-      "rasterize": ("tile rasterizer (bin count + scan + bin fill + tile "
-                    "raster/resolve)", bytes_raster),
+      "splat_source": ("source_kernel<4> (splat / interpolate / transform "
+                       "VJP, gradient scatter)", bytes_source),
+      "splat_target": ("target_kernel<4> (interpolate + splat + over + L2 "
+                       "gradient)", bytes_target),
   }
   dom = max(per_stage, key=lambda k: stage_ms[k])
   name, nbytes = per_stage[dom]
-  achieved = nbytes / (stage_ms[dom] * 1e-3) / 1e9
+  launch_ms = stage_ms[dom] / ng
+  launch_bytes = nbytes / ng
+  achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
   roofline = dict(bound="hbm", kernel=name, achieved=round(achieved, 1),
                   peak=HBM_PEAK_GBS, unit="GB/s",
-                  frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                  algorithmic_bytes=nbytes, avg_ms=round(stage_ms[dom], 4))
+                  frac=round(achieved / HBM_PEAK_GBS, 4),
+                  traffic=measured_traffic(dom, ng),
+                  algorithmic_bytes_per_launch=int(launch_bytes),
+                  avg_launch_ms=round(launch_ms, 4), launches_per_step=ng)
 
   # Plain path: one render_triangles per view + autograd.
   for _ in range(1):
