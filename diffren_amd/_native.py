@@ -19,6 +19,7 @@ _c_size = ctypes.c_size_t
 
 
 RASTER_SPARSE = 1
+MLP_ALBEDO_MODULATE = 1
 
 
 class RasterConfig(ctypes.Structure):
@@ -99,10 +100,12 @@ SIGNATURES = {
     "dr_mlp_packed_bytes": (_c_size, []),
     "dr_mlp_pack": (_c_int, [_c_ptr, _c_ptr, _c_ptr]),
     "dr_mlp_forward": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_i64, _c_ptr,
-                                _c_int, _c_ptr]),
+                                _c_ptr, _c_int, _c_int, _c_ptr]),
+    "dr_mlp_train_workspace_size": (_c_size, [_c_int]),
     "dr_mlp_train_step": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr,
-                                   _c_i64, ctypes.c_float, _c_ptr, _c_ptr,
-                                   _c_int, _c_ptr]),
+                                   _c_i64, ctypes.c_float, _c_ptr, _c_int,
+                                   _c_ptr, _c_ptr, _c_int, _c_ptr, _c_size,
+                                   _c_ptr]),
     "dr_mlp_adam": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_int,
                              ctypes.c_float, ctypes.c_float, ctypes.c_float,
                              ctypes.c_float, _c_ptr]),

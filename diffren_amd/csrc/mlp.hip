@@ -111,17 +111,43 @@ __global__ void pack_kernel(const float* __restrict__ p,
 // ---------------------------------------------------------------------------
 // Forward building blocks.
 
-// Input chunk as the layer-1 B operand: pixel col, features 8h..8h+7.
-__device__ __forceinline__ bf16x8 input_operand(const float* __restrict__ x,
-                                                int64_t p, int64_t N, int h) {
+// Raw inputs of the layer-1 B operand: pixel p, features 8h..8h+7 (zero
+// beyond the 12 features or the last pixel).
+__device__ __forceinline__ void load_inputs(const float* __restrict__ x,
+                                            int64_t p, int64_t N, int h,
+                                            float raw[8]) {
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int f = 8 * h + j;
+    raw[j] = (p < N && f < kIn) ? x[p * kIn + f] : 0.0f;
+  }
+}
+
+// (raw - shift) * scale as bfloat16; nrm = [shift 12 | scale 12] in LDS.
+__device__ __forceinline__ bf16x8 input_operand(const float raw[8],
+                                                const float* nrm, int h) {
   bf16x8 b;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     const int f = 8 * h + j;
-    const float v = (p < N && f < kIn) ? x[p * kIn + f] : 0.0f;
+    const float v = f < kIn ? (raw[j] - nrm[f]) * nrm[kIn + f] : 0.0f;
     b[j] = static_cast<__bf16>(v);
   }
   return b;
+}
+
+__device__ __forceinline__ void stage_norm(const float* __restrict__ norm,
+                                           float* nrm) {
+  for (int i = threadIdx.x; i < 2 * kIn; i += blockDim.x)
+    nrm[i] = norm ? norm[i] : (i < kIn ? 0.0f : 1.0f);
+}
+
+// Albedo (features 9..11) of pixel p for the output lanes.
+__device__ __forceinline__ void load_albedo(const float* __restrict__ x,
+                                            int64_t p, int64_t N,
+                                            float alb[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) alb[c] = p < N ? x[p * kIn + 9 + c] : 0.0f;
 }
 
 // ReLU of two accumulator tiles as the next layer's four B operands.
@@ -154,27 +180,43 @@ __device__ __forceinline__ f32x16 bias_init(const float* sb, int layer, int m,
 }
 
 // ---------------------------------------------------------------------------
-// Inference: persistent waves, all weights in registers.
+// Inference: persistent waves; weights and biases in LDS (keeping the 30
+// operand blocks in registers would leave one wave per SIMD); the next
+// chunk's inputs are loaded while the current chunk runs on the matrix cores.
 
 __global__ __launch_bounds__(kBlock) void forward_kernel(
     const bf16x8* __restrict__ wf, const float* __restrict__ params,
-    const float* __restrict__ x, int64_t N, float* __restrict__ y) {
+    const float* __restrict__ x, int64_t N, float* __restrict__ y,
+    const float* __restrict__ norm, int flags) {
   __shared__ float sb[5 * 64];
+  __shared__ float nrm[2 * kIn];
+  __shared__ bf16x8 sw[kFwdBlocks * 64];
   stage_bias(params, sb);
+  stage_norm(norm, nrm);
+  for (int i = threadIdx.x; i < kFwdBlocks * 64; i += kBlock) sw[i] = wf[i];
   __syncthreads();
+  const bool modulate = (flags & DR_MLP_ALBEDO_MODULATE) != 0;
   const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
-  bf16x8 w[kFwdBlocks];
-#pragma unroll
-  for (int i = 0; i < kFwdBlocks; ++i) w[i] = wf[i * 64 + lane];
   const int64_t waves = int64_t(gridDim.x) * kWaves;
-  for (int64_t chunk = int64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6);
-       chunk * 32 < N; chunk += waves) {
+  int64_t chunk = int64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+  float raw[8], alb[3];
+  load_inputs(x, chunk * 32 + col, N, h, raw);
+  if (modulate) load_albedo(x, chunk * 32 + col, N, alb);
+  for (; chunk * 32 < N; chunk += waves) {
+    // Keeps the weight reads from LDS inside the loop (hoisting them would
+    // pin all 30 operand blocks in registers again).
+    __asm__ volatile("" ::: "memory");
     const int64_t p = chunk * 32 + col;
-    const bf16x8 b0 = input_operand(x, p, N, h);
+    const bf16x8 b0 = input_operand(raw, nrm, h);
+    float a[3] = {alb[0], alb[1], alb[2]};
+    // Prefetch the next chunk (its loads are in bounds or masked).
+    load_inputs(x, (chunk + waves) * 32 + col, N, h, raw);
+    if (modulate) load_albedo(x, (chunk + waves) * 32 + col, N, alb);
     f32x16 acc[2];
 #pragma unroll
     for (int m = 0; m < 2; ++m)
-      acc[m] = mfma(w[kFwdBase[0] + m], b0, bias_init(sb, 0, m, h));
+      acc[m] = mfma(sw[(kFwdBase[0] + m) * 64 + lane], b0,
+                    bias_init(sb, 0, m, h));
     bf16x8 nb[4];
 #pragma unroll
     for (int layer = 1; layer < 4; ++layer) {
@@ -184,18 +226,19 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(
         f32x16 c = bias_init(sb, layer, m, h);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
-          c = mfma(w[kFwdBase[layer] + 4 * m + s], nb[s], c);
+          c = mfma(sw[(kFwdBase[layer] + 4 * m + s) * 64 + lane], nb[s], c);
         acc[m] = c;
       }
     }
     relu_operands(acc, nb);
     f32x16 c = bias_init(sb, 4, 0, h);
 #pragma unroll
-    for (int s = 0; s < 4; ++s) c = mfma(w[kFwdBase[4] + s], nb[s], c);
+    for (int s = 0; s < 4; ++s)
+      c = mfma(sw[(kFwdBase[4] + s) * 64 + lane], nb[s], c);
     if (h == 0 && p < N) {
-      y[p * 3 + 0] = c[0];
-      y[p * 3 + 1] = c[1];
-      y[p * 3 + 2] = c[2];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+        y[p * 3 + r] = modulate ? a[r] * c[r] : c[r];
     }
   }
 }
@@ -246,8 +289,8 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
     const bf16x8* __restrict__ wf, const bf16x8* __restrict__ wb,
     const float* __restrict__ params, const float* __restrict__ x,
     const float* __restrict__ target, const float* __restrict__ mask,
-    int64_t N, float scale, float* __restrict__ grad,
-    float* __restrict__ loss) {
+    int64_t N, float scale, float* __restrict__ partials,
+    const float* __restrict__ norm, int flags) {
   extern __shared__ __attribute__((aligned(16))) float lds[];
   float* g = lds;                       // [kParams] gradient accumulators
   float* sb = g + kParams + 1;          // [5 * 64] biases
@@ -257,9 +300,13 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
   const int wave = threadIdx.x >> 6;
   __bf16* act = act_all + wave * (kActWave + kDltWave);
   __bf16* dlt = act + kActWave;
+  __shared__ float nrm[2 * kIn];
+  __shared__ float wave_loss[kWaves];
   for (int i = threadIdx.x; i < kParams; i += kBlock) g[i] = 0.0f;
   stage_bias(params, sb);
+  stage_norm(norm, nrm);
   __syncthreads();
+  const bool modulate = (flags & DR_MLP_ALBEDO_MODULATE) != 0;
 
   const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
   const int64_t waves = int64_t(gridDim.x) * kWaves;
@@ -268,7 +315,9 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
        chunk += waves) {
     const int64_t p = chunk * 32 + col;
     // ---- forward, keeping activations
-    const bf16x8 b0 = input_operand(x, p, N, h);
+    float raw[8];
+    load_inputs(x, p, N, h, raw);
+    const bf16x8 b0 = input_operand(raw, nrm, h);
 #pragma unroll
     for (int j = 0; j < 8; ++j) act[(8 * h + j) * 32 + col] = b0[j];
     f32x16 acc[2];
@@ -303,11 +352,14 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
     for (int r = 0; r < 16; ++r) { d[0][r] = 0.0f; d[1][r] = 0.0f; }
     if (h == 0 && p < N) {
       const float mk = mask ? mask[p] : 1.0f;
+      float alb[3] = {1.0f, 1.0f, 1.0f};
+      if (modulate) load_albedo(x, p, N, alb);
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
-        const float e = y[r] - target[p * 3 + r];
+        // image = albedo * network output when modulating.
+        const float e = alb[r] * y[r] - target[p * 3 + r];
         partial += mk * (e * e) * scale;
-        d[0][r] = ((2.0f * e) * scale) * mk;
+        d[0][r] = (((2.0f * e) * scale) * mk) * alb[r];
       }
     }
     // ---- backward, layer 5 .. 1
@@ -381,13 +433,30 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
       lds_fence();  // dlt is rewritten by the next layer
     }
   }
-  if (loss) {
-    for (int off = 32; off > 0; off >>= 1) partial += __shfl_xor(partial, off);
-    if (lane == 0 && partial != 0.0f) unsafeAtomicAdd(loss, partial);
-  }
+  for (int off = 32; off > 0; off >>= 1) partial += __shfl_xor(partial, off);
+  if (lane == 0) wave_loss[wave] = partial;
   __syncthreads();
-  for (int i = threadIdx.x; i < kParams; i += kBlock)
-    if (g[i] != 0.0f) unsafeAtomicAdd(&grad[i], g[i]);
+  // This block's gradient and loss, stored (reduced over blocks in a fixed
+  // order by reduce_kernel: no same-address atomics across blocks).
+  float* out = partials + int64_t(blockIdx.x) * (kParams + 1);
+  for (int i = threadIdx.x; i < kParams; i += kBlock) out[i] = g[i];
+  if (threadIdx.x == 0) {
+    float t = 0.0f;
+    for (int w = 0; w < kWaves; ++w) t += wave_loss[w];
+    out[kParams] = t;
+  }
+}
+
+// grad[i] += sum over blocks of partials[b][i]; *loss += sum of the losses.
+__global__ __launch_bounds__(256) void reduce_kernel(
+    const float* __restrict__ partials, int nblocks, float* __restrict__ grad,
+    float* __restrict__ loss) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i > kParams) return;
+  float t = 0.0f;
+  for (int b = 0; b < nblocks; ++b) t += partials[int64_t(b) * (kParams + 1) + i];
+  if (i < kParams) grad[i] += t;
+  else if (loss) *loss += t;
 }
 
 // Adam on float32 master weights (Kingma & Ba, 2015).
@@ -437,41 +506,61 @@ int dr_mlp_pack(const float* params, void* packed, dr_stream_t stream) {
 }
 
 int dr_mlp_forward(const void* packed, const float* params, const float* x,
-                   int64_t num_pixels, float* y, int32_t num_blocks,
-                   dr_stream_t stream) {
+                   int64_t num_pixels, float* y, const float* input_norm,
+                   int32_t flags, int32_t num_blocks, dr_stream_t stream) {
   DR_CHECK_ARG(num_pixels >= 0, "negative size");
   if (num_pixels == 0) return DR_OK;
   DR_CHECK_ARG(packed && params && x && y, "null pointer");
   const int64_t chunks = (num_pixels + 31) / 32;
-  int64_t nb = num_blocks > 0 ? num_blocks : 1024;
+  int64_t nb = num_blocks > 0 ? num_blocks : 2048;
   nb = std::min<int64_t>(nb, (chunks + dr::kWaves - 1) / dr::kWaves);
   dr::forward_kernel<<<unsigned(nb), dr::kBlock, 0,
                        static_cast<hipStream_t>(stream)>>>(
-      static_cast<const dr::bf16x8*>(packed), params, x, num_pixels, y);
+      static_cast<const dr::bf16x8*>(packed), params, x, num_pixels, y,
+      input_norm, flags);
   DR_CHECK_LAUNCH("mlp forward");
   return DR_OK;
 }
 
+size_t dr_mlp_train_workspace_size(int32_t num_blocks) {
+  const int nb = num_blocks > 0 ? num_blocks : 256;
+  return size_t(nb) * (dr::kParams + 1) * sizeof(float);
+}
+
 int dr_mlp_train_step(const void* packed, const float* params, const float* x,
                       const float* target, const float* mask,
-                      int64_t num_pixels, float loss_scale, float* grad,
-                      float* loss, int32_t num_blocks, dr_stream_t stream) {
+                      int64_t num_pixels, float loss_scale,
+                      const float* input_norm, int32_t flags, float* grad,
+                      float* loss, int32_t num_blocks, void* workspace,
+                      size_t workspace_bytes, dr_stream_t stream) {
   DR_CHECK_ARG(num_pixels >= 0, "negative size");
   if (num_pixels == 0) return DR_OK;
   DR_CHECK_ARG(packed && params && x && target && grad, "null pointer");
+  DR_CHECK_ARG(workspace != nullptr, "workspace is null");
+  const int nb0 = num_blocks > 0 ? num_blocks : 256;
+  if (workspace_bytes < dr_mlp_train_workspace_size(nb0))
+    return dr::fail(dr::kWorkspaceTooSmall, "mlp train workspace too small");
+  static bool lds_set = false;  // one attribute call per process
   const size_t lds = dr::train_lds_bytes();
-  DR_CHECK_HIP(hipFuncSetAttribute(
-      reinterpret_cast<const void*>(&dr::train_kernel),
-      hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  if (!lds_set) {
+    DR_CHECK_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&dr::train_kernel),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    lds_set = true;
+  }
   const int64_t chunks = (num_pixels + 31) / 32;
-  int64_t nb = num_blocks > 0 ? num_blocks : 256;
-  nb = std::min<int64_t>(nb, (chunks + dr::kWaves - 1) / dr::kWaves);
+  const int nb = int(std::min<int64_t>(nb0, (chunks + dr::kWaves - 1) /
+                                                dr::kWaves));
   const auto* f = static_cast<const dr::bf16x8*>(packed);
-  dr::train_kernel<<<unsigned(nb), dr::kBlock, lds,
-                     static_cast<hipStream_t>(stream)>>>(
+  float* partials = static_cast<float*>(workspace);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  dr::train_kernel<<<unsigned(nb), dr::kBlock, lds, s>>>(
       f, f + dr::kFwdBlocks * 64, params, x, target, mask, num_pixels,
-      loss_scale, grad, loss);
+      loss_scale, partials, input_norm, flags);
   DR_CHECK_LAUNCH("mlp train step");
+  dr::reduce_kernel<<<(dr::kParams + 1 + 255) / 256, 256, 0, s>>>(
+      partials, nb, grad, loss);
+  DR_CHECK_LAUNCH("mlp train reduce");
   return DR_OK;
 }
 

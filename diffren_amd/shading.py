@@ -70,10 +70,30 @@ def random_lights(num_lights, seed=0, r_min=4.0, r_max=5.0, center=(0, 0, 0)):
   return np.concatenate([pos, inten], 1).astype(np.float32)
 
 
-class MLPShader:
-  """The learned deferred shader and its on-GPU training state."""
+MLP_SIZES = (12, 64, 64, 64, 64, 3)
 
-  def __init__(self, params, device):
+
+def init_params(seed=0):
+  """He-normal weights, zero biases, flat in the kernels' order:
+  W1[12,64] b1 W2[64,64] b2 W3 b3 W4 b4 W5[64,3] b5."""
+  rng = np.random.default_rng(seed)
+  parts = []
+  for k, n in zip(MLP_SIZES[:-1], MLP_SIZES[1:]):
+    parts.append((rng.normal(size=k * n) * np.sqrt(2.0 / k)))
+    parts.append(np.zeros(n))
+  return np.concatenate(parts).astype(np.float32)
+
+
+class MLPShader:
+  """The learned deferred shader and its on-GPU training state.
+
+  input_norm: optional [24] (shift 12 | scale 12): the network sees
+  (x - shift) * scale.  albedo_modulate: the image is albedo * network output
+  (the network learns the light response; G-buffer albedo is x[..., 9:12]).
+  """
+
+  def __init__(self, params, device, input_norm=None, albedo_modulate=False,
+               train_blocks=256):
     lib = _native.lib()
     n = lib.dr_mlp_num_params()
     p = torch.as_tensor(np.asarray(params, np.float32), device=device)
@@ -86,8 +106,23 @@ class MLPShader:
     self.m = torch.zeros_like(self.params)
     self.v = torch.zeros_like(self.params)
     self.loss = torch.zeros(1, dtype=torch.float32, device=device)
+    self.norm = None if input_norm is None else torch.as_tensor(
+        np.asarray(input_norm, np.float32), device=device).contiguous()
+    self.flags = _native.MLP_ALBEDO_MODULATE if albedo_modulate else 0
+    self.train_blocks = int(train_blocks)
+    self.train_ws = _native.workspace(
+        lib.dr_mlp_train_workspace_size(self.train_blocks), device)
     self.step_count = 0
     self.pack()
+
+  @staticmethod
+  def normalization(x, eps=1e-6):
+    """[24] shift/scale from features x [N, 12] (mean and 1/std)."""
+    x = x.reshape(-1, 12).to(torch.float64)
+    mean = x.mean(0)
+    std = x.std(0)
+    return torch.cat([mean, 1.0 / (std + eps)]).to(torch.float32).cpu(
+        ).numpy()
 
   def pack(self):
     _native.check(_native.lib().dr_mlp_pack(_native.ptr(self.params),
@@ -104,11 +139,12 @@ class MLPShader:
         _native.lib().dr_mlp_forward(_native.ptr(self.packed),
                                      _native.ptr(self.params), _native.ptr(x),
                                      x.numel() // 12, _native.ptr(y),
+                                     _native.ptr(self.norm), self.flags,
                                      num_blocks, _native.stream_handle()),
         "dr_mlp_forward")
     return y
 
-  def loss_and_grad(self, x, target, mask=None, scale=None, num_blocks=0):
+  def loss_and_grad(self, x, target, mask=None, scale=None):
     """Clears and fills self.loss / self.grad for one pass over x."""
     x = x.contiguous()
     n = x.numel() // 12
@@ -123,7 +159,9 @@ class MLPShader:
         _native.lib().dr_mlp_train_step(
             _native.ptr(self.packed), _native.ptr(self.params),
             _native.ptr(x), _native.ptr(t), _native.ptr(mk), n, float(scale),
-            _native.ptr(self.grad), _native.ptr(self.loss), num_blocks,
+            _native.ptr(self.norm), self.flags, _native.ptr(self.grad),
+            _native.ptr(self.loss), self.train_blocks,
+            _native.ptr(self.train_ws), self.train_ws.numel(),
             _native.stream_handle()), "dr_mlp_train_step")
     return self.loss
 

@@ -340,18 +340,30 @@ size_t dr_mlp_packed_bytes(void);
  * backward orders); call after every parameter update. */
 int dr_mlp_pack(const float* params, void* packed, dr_stream_t stream);
 
+/* Input normalisation and output options of the MLP entry points:
+ * input_norm (device, may be NULL = identity) holds [shift 12 | scale 12]
+ * and the network sees (x - shift) * scale; with DR_MLP_ALBEDO_MODULATE the
+ * image is albedo * network output, albedo being x[:, 9:12] (raw). */
+enum { DR_MLP_ALBEDO_MODULATE = 1 };
+
 /* y [num_pixels, 3] = MLP(x [num_pixels, 12]).  num_blocks <= 0: default. */
 int dr_mlp_forward(const void* packed, const float* params, const float* x,
-                   int64_t num_pixels, float* y, int32_t num_blocks,
-                   dr_stream_t stream);
+                   int64_t num_pixels, float* y, const float* input_norm,
+                   int32_t flags, int32_t num_blocks, dr_stream_t stream);
+
+/* Workspace bytes of dr_mlp_train_step for num_blocks (<= 0: default). */
+size_t dr_mlp_train_workspace_size(int32_t num_blocks);
 
 /* One training pass: loss += loss_scale * sum_p mask[p] |y_p - target_p|^2
  * and grad += d loss / d params (float32, accumulated: the caller clears
- * grad and loss).  mask [num_pixels] may be NULL (all ones). */
+ * grad and loss; per-block partials are summed in a fixed order, so the
+ * result is deterministic).  mask [num_pixels] may be NULL (all ones). */
 int dr_mlp_train_step(const void* packed, const float* params, const float* x,
                       const float* target, const float* mask,
-                      int64_t num_pixels, float loss_scale, float* grad,
-                      float* loss, int32_t num_blocks, dr_stream_t stream);
+                      int64_t num_pixels, float loss_scale,
+                      const float* input_norm, int32_t flags, float* grad,
+                      float* loss, int32_t num_blocks, void* workspace,
+                      size_t workspace_bytes, dr_stream_t stream);
 
 /* Adam update of params with moments m, v (all float32, num_params);
  * step counts from 1. */

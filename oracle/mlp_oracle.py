@@ -58,10 +58,20 @@ def bf16(x):
   return u.astype(np.uint32).view(np.float32)
 
 
-def forward(params, x, keep=False):
-  """x [N,12] f32 -> y [N,3] f32 (and per-layer activations if keep)."""
+def normalized(x, norm):
+  """(x - shift) * scale in float32 (norm = [shift 12 | scale 12] or None)."""
+  x = np.asarray(x, np.float32)
+  if norm is None:
+    return x
+  norm = np.asarray(norm, np.float32)
+  return ((x - norm[:12]) * norm[12:]).astype(np.float32)
+
+
+def forward(params, x, keep=False, norm=None, modulate=False):
+  """x [N,12] f32 -> y [N,3] f32 (and per-layer activations if keep).
+  With modulate the image is albedo x[:, 9:12] * network output."""
   layers = unpack(params)
-  a = bf16(x).astype(np.float64)
+  a = bf16(normalized(x, norm)).astype(np.float64)
   acts, pre = [a], []
   for i, (W, b) in enumerate(layers):
     z = a @ bf16(W).astype(np.float64) + b.astype(np.float64)
@@ -71,17 +81,21 @@ def forward(params, x, keep=False):
       a = bf16(np.maximum(z, 0.0)).astype(np.float64)
       acts.append(a)
   y = pre[-1]
+  if modulate:
+    y = (np.asarray(x, np.float32)[:, 9:12] * y).astype(np.float32)
   return (y, acts, pre) if keep else y
 
 
-def loss_and_grad(params, x, target, mask, scale):
+def loss_and_grad(params, x, target, mask, scale, norm=None, modulate=False):
   """L = scale * sum(mask * (y - t)^2); returns (L, dL/dparams flat f32)."""
   layers = unpack(params)
-  y, acts, pre = forward(params, x, keep=True)
+  y, acts, pre = forward(params, x, keep=True, norm=norm, modulate=modulate)
   m = np.asarray(mask, np.float32)[:, None]
   d = (y - target).astype(np.float64)
   loss = float(scale * np.sum(m * d * d))
   delta = (2.0 * d * scale * m).astype(np.float32)
+  if modulate:
+    delta = (delta * np.asarray(x, np.float32)[:, 9:12]).astype(np.float32)
   grads = [None] * 5
   for i in range(4, -1, -1):
     W, _ = layers[i]

@@ -62,7 +62,21 @@ def test_mlp_forward_matches_oracle(device):
   assert helpers.rel_err(y, want) <= 2e-3
 
 
-def test_mlp_gradient_matches_oracle(device):
+@pytest.mark.parametrize("modulate", [False, True])
+def test_mlp_forward_normalised_modulated(modulate, device):
+  rng = np.random.default_rng(5)
+  p = mo.init_params(6)
+  x = (rng.normal(size=(3000, 12)) * 3 + 1).astype(np.float32)
+  norm = np.concatenate([x.mean(0), 1.0 / x.std(0)]).astype(np.float32)
+  net = shading.MLPShader(p, device, input_norm=norm,
+                          albedo_modulate=modulate)
+  y = net.forward(torch.as_tensor(x, device=device)).cpu().numpy()
+  want = mo.forward(p, x, norm=norm, modulate=modulate)
+  assert helpers.rel_err(y, want) <= 2e-3
+
+
+@pytest.mark.parametrize("modulate", [False, True])
+def test_mlp_gradient_matches_oracle(modulate, device):
   rng = np.random.default_rng(1)
   p = mo.init_params(4)
   n = 4099
@@ -70,11 +84,15 @@ def test_mlp_gradient_matches_oracle(device):
   t = rng.normal(size=(n, 3)).astype(np.float32)
   mask = (rng.uniform(size=n) > 0.3).astype(np.float32)
   scale = 1.0 / (3 * n)
-  net = shading.MLPShader(p, device)
+  norm = np.concatenate([np.full(12, 0.1), np.full(12, 0.9)]).astype(
+      np.float32)
+  net = shading.MLPShader(p, device, input_norm=norm,
+                          albedo_modulate=modulate, train_blocks=7)
   net.loss_and_grad(torch.as_tensor(x, device=device),
                     torch.as_tensor(t, device=device),
                     torch.as_tensor(mask, device=device), scale)
-  loss, g = mo.loss_and_grad(p, x, t, mask, scale)
+  loss, g = mo.loss_and_grad(p, x, t, mask, scale, norm=norm,
+                             modulate=modulate)
   got = net.grad.cpu().numpy()
   assert abs(net.loss.item() - loss) <= 1e-3 * abs(loss)
   assert np.linalg.norm(got - g) <= 1e-2 * np.linalg.norm(g)
