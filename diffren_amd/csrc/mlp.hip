@@ -246,43 +246,179 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(
 // ---------------------------------------------------------------------------
 // Training step: forward, masked L2 loss, backward, dW/db.
 //
-// LDS per wave: activations a0 [16][32], a1..a4 [64][32] and the current
-// delta [64][32], all bfloat16 as [unit][pixel] so that 8 consecutive pixels
-// of one unit -- an A or B operand with K = pixels -- are one 16-byte read.
-// Per block: float32 gradient accumulators for all parameters.
+// A block of four waves works on rounds of 128 pixels, 32 per wave.  Each
+// wave runs the forward and the input-gradient (d in = W d out) chain of its
+// own 32 pixels on the matrix cores, as in inference, and writes the layer
+// inputs and output deltas of those pixels to LDS as bfloat16 [unit][pixel].
+// The weight gradients dW_l[in][out] = sum_pixels a_l[in] delta_l[out] are
+// 32x32 tiles with the 128 pixels of a round as K; every tile has one owner
+// wave that keeps it in accumulator registers across all rounds of the block
+// (layers 1..3: wave w owns out tile w&1, in tile w>>1; layer 0: waves 0, 1;
+// layer 4: waves 2, 3), so no gradient is ever added through memory until
+// the block writes its partial sums once at the end.  Bias gradients are the
+// row sums of the delta operands the owners of in tile 0 read anyway.
+//
+// LDS: the 56 packed weight blocks, the five layer inputs a0 [16][128],
+// a1..a4 [64][128] and two delta buffers [64][128] (layer l uses buffer
+// l&1; one barrier per layer orders them).  Row r of an array keeps its
+// 16-byte groups of 8 pixels XOR-swizzled by r&15: operand reads (32 rows,
+// one group) and tile writes (rows r and r+4 from the two lane halves) are
+// then spread over all banks.
 
-constexpr int kActWave = (16 + 4 * 64) * 32;  // bf16 elements
-constexpr int kDltWave = 64 * 32;
+constexpr int kRound = 32 * kWaves;  // pixels per block round
+constexpr int kActRows = 16 + 4 * kHid;
+constexpr size_t kTrainLdsBytes =
+    size_t(kFwdBlocks + kBwdBlocks) * 64 * 16 +      // weights
+    size_t(kActRows + 2 * kHid) * kRound * 2;        // a0..a4, 2 deltas
 
-__device__ __forceinline__ void lds_fence() {
-  // Every lane's LDS writes complete before any lane reads them back.
-  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-  __builtin_amdgcn_wave_barrier();
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ int swz(int row, int px) {
+  return row * kRound + ((((px >> 3) ^ (row & 15)) << 3) | (px & 7));
 }
 
-__device__ __forceinline__ __bf16* act_ptr(__bf16* base, int layer) {
-  // layer 0: a0 [16][32]; layer i>0: a_i [64][32]
-  return layer == 0 ? base : base + 16 * 32 + (layer - 1) * 64 * 32;
-}
-
-__device__ __forceinline__ void store_tile(__bf16* arr, const f32x16& c, int m,
-                                           int h, int col, bool relu) {
+__device__ __forceinline__ void put_tile(__bf16* arr, const f32x16& c, int m,
+                                         int h, int px, bool relu) {
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const float v = relu ? fmaxf(c[r], 0.0f) : c[r];
-    arr[c_unit(m, r, h) * 32 + col] = static_cast<__bf16>(v);
+    arr[swz(c_unit(m, r, h), px)] = static_cast<__bf16>(v);
   }
 }
 
-__device__ __forceinline__ bf16x8 lds_operand(const __bf16* arr, int row,
-                                              int nrows, int s, int h) {
+// Operand with K = pixels 16s..16s+15 of the round: lane row, 8 pixels.
+__device__ __forceinline__ bf16x8 px_operand(const __bf16* arr, int row,
+                                             int nrows, int s, int h) {
   if (row >= nrows) {
     bf16x8 z;
 #pragma unroll
     for (int j = 0; j < 8; ++j) z[j] = static_cast<__bf16>(0.0f);
     return z;
   }
-  return *reinterpret_cast<const bf16x8*>(arr + row * 32 + 16 * s + 8 * h);
+  return *reinterpret_cast<const bf16x8*>(
+      arr + row * kRound + (((2 * s + h) ^ (row & 15)) << 3));
+}
+
+__device__ __forceinline__ float row_sum(const bf16x8& a, float acc) {
+  const bf16x2 one = {static_cast<__bf16>(1.0f), static_cast<__bf16>(1.0f)};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const bf16x2 v = {a[2 * j], a[2 * j + 1]};
+    acc = __builtin_amdgcn_fdot2_f32_bf16(v, one, acc, false);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ uint32_t relu_bits(const f32x16 acc[2]) {
+  uint32_t b = 0;
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      b |= (acc[m][r] > 0.0f ? 1u : 0u) << (16 * m + r);
+  return b;
+}
+
+// dW_L tile (out tile m, in tile n) += delta_L a_L^T over the round's pixels;
+// with `bias`, also the delta row sums.
+template <int L>
+__device__ __forceinline__ void weight_grad(f32x16& acc, float& bacc,
+                                            const __bf16* dlt,
+                                            const __bf16* ain, int m, int n,
+                                            int h, int col, bool bias) {
+  constexpr int out_rows = kOutSize[L];
+  constexpr int in_rows = L == 0 ? 16 : kHid;
+#pragma unroll
+  for (int s = 0; s < kRound / 16; ++s) {
+    const bf16x8 a = px_operand(dlt, 32 * m + col, out_rows, s, h);
+    const bf16x8 b = px_operand(ain, 32 * n + col, in_rows, s, h);
+    acc = mfma(a, b, acc);
+    if (bias) bacc = row_sum(a, bacc);
+  }
+}
+
+// d a_L = W_L d out, as the delta of layer L-1 (ReLU' from its forward
+// pre-activation signs, bit 16m + r of `bits`).
+template <int L>
+__device__ __forceinline__ void input_grad(const bf16x8* sw, const f32x16 d[2],
+                                           uint32_t bits, int lane,
+                                           f32x16 nd[2]) {
+  bf16x8 db[4];
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        db[2 * m + s][j] = static_cast<__bf16>(d[m][8 * s + j]);
+  const bf16x8* wb = sw + kFwdBlocks * 64;
+#pragma unroll
+  for (int m = 0; m < 2; ++m) {
+    f32x16 c;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) c[r] = 0.0f;
+    if (L == 4) {
+      c = mfma(wb[(kBwdBase[4] + m) * 64 + lane], db[0], c);
+    } else {
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+        c = mfma(wb[(kBwdBase[L] + 4 * m + s) * 64 + lane], db[s], c);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      c[r] = (bits >> (16 * m + r)) & 1u ? c[r] : 0.0f;
+    nd[m] = c;
+  }
+}
+
+// One backward layer L: barrier (delta_L complete in dlt[L&1]), the owned
+// dW_L tile, then delta_{L-1} (computed before the barrier) to dlt[(L-1)&1].
+template <int L>
+__device__ __forceinline__ void backward_layer(
+    f32x16 acc[4], float bacc[5], __bf16* act, __bf16* dlt, f32x16 d[2],
+    const f32x16 nd[2], int wave, int h, int col, int px) {
+  __syncthreads();
+  const __bf16* ain = L == 0 ? act : act + (16 + (L - 1) * kHid) * kRound;
+  const __bf16* dl = dlt + (L & 1) * kHid * kRound;
+  if (L >= 1 && L <= 3) {
+    weight_grad<L>(acc[L - 1], bacc[L], dl, ain, wave & 1, wave >> 1, h, col,
+                   (wave >> 1) == 0);
+  } else if (L == 0) {
+    if (wave < 2) weight_grad<0>(acc[3], bacc[0], dl, ain, wave, 0, h, col,
+                                 true);
+  } else {
+    if (wave >= 2) weight_grad<4>(acc[3], bacc[4], dl, ain, 0, wave - 2, h,
+                                  col, wave == 2);
+  }
+  if (L > 0) {
+    __bf16* next = dlt + ((L - 1) & 1) * kHid * kRound;
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      put_tile(next, nd[m], m, h, px, false);
+      d[m] = nd[m];
+    }
+  }
+}
+
+// Block partial sums of one dW tile and (if kept) of its bias rows.
+template <int L>
+__device__ __forceinline__ void write_tile(float* out, const f32x16& acc,
+                                           int m, int n, int h, int col) {
+  const int in = 32 * n + col;
+  if (in >= kInSize[L]) return;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int u = c_unit(m, r, h);
+    if (u < kOutSize[L]) out[kW[L] + in * kOutSize[L] + u] = acc[r];
+  }
+}
+
+template <int L>
+__device__ __forceinline__ void write_bias(float* out, float bacc, int m,
+                                           int h, int col) {
+  const float t = bacc + __shfl_xor(bacc, 32);
+  const int u = 32 * m + col;
+  if (h == 0 && u < kOutSize[L]) out[kB[L] + u] = t;
 }
 
 __global__ __launch_bounds__(kBlock) void train_kernel(
@@ -291,61 +427,78 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
     const float* __restrict__ target, const float* __restrict__ mask,
     int64_t N, float scale, float* __restrict__ partials,
     const float* __restrict__ norm, int flags) {
-  extern __shared__ __attribute__((aligned(16))) float lds[];
-  float* g = lds;                       // [kParams] gradient accumulators
-  float* sb = g + kParams + 1;          // [5 * 64] biases
-  // kParams + 1 + 5 * 64 = 13828 floats: the bf16 arrays start 16-byte
-  // aligned, as their 16-byte operand reads need.
-  __bf16* act_all = reinterpret_cast<__bf16*>(sb + 5 * 64);
-  const int wave = threadIdx.x >> 6;
-  __bf16* act = act_all + wave * (kActWave + kDltWave);
-  __bf16* dlt = act + kActWave;
+  extern __shared__ __attribute__((aligned(16))) bf16x8 sw[];
+  __bf16* act = reinterpret_cast<__bf16*>(sw + (kFwdBlocks + kBwdBlocks) * 64);
+  __bf16* dlt = act + kActRows * kRound;
+  __shared__ float sb[5 * 64];
   __shared__ float nrm[2 * kIn];
   __shared__ float wave_loss[kWaves];
-  for (int i = threadIdx.x; i < kParams; i += kBlock) g[i] = 0.0f;
+  for (int i = threadIdx.x; i < kFwdBlocks * 64; i += kBlock) sw[i] = wf[i];
+  for (int i = threadIdx.x; i < kBwdBlocks * 64; i += kBlock)
+    sw[kFwdBlocks * 64 + i] = wb[i];
   stage_bias(params, sb);
   stage_norm(norm, nrm);
-  __syncthreads();
   const bool modulate = (flags & DR_MLP_ALBEDO_MODULATE) != 0;
 
-  const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
-  const int64_t waves = int64_t(gridDim.x) * kWaves;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  f32x16 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+  float bacc[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   float partial = 0.0f;
-  for (int64_t chunk = int64_t(blockIdx.x) * kWaves + wave; chunk * 32 < N;
-       chunk += waves) {
-    const int64_t p = chunk * 32 + col;
-    // ---- forward, keeping activations
+
+  for (int64_t q = blockIdx.x; q * kRound < N; q += gridDim.x) {
+    // The previous round's owners are done with a*, delta buffers.
+    __syncthreads();
+    // Lane coordinates, opaque to the compiler so that it derives the LDS
+    // addresses inside the loop instead of keeping hundreds of them live.
+    int h = lane >> 5, col = lane & 31;
+    __asm__ volatile("" : "+v"(h), "+v"(col));
+    const int px = 32 * wave + col;  // pixel within the round
+    const int64_t p = q * kRound + px;
+    // ---- forward, keeping layer inputs in LDS and ReLU signs in bits
     float raw[8];
     load_inputs(x, p, N, h, raw);
     const bf16x8 b0 = input_operand(raw, nrm, h);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) act[(8 * h + j) * 32 + col] = b0[j];
-    f32x16 acc[2];
+    for (int j = 0; j < 8; ++j) act[swz(8 * h + j, px)] = b0[j];
+    f32x16 a[2];
 #pragma unroll
     for (int m = 0; m < 2; ++m)
-      acc[m] = mfma(wf[(kFwdBase[0] + m) * 64 + lane], b0,
-                    bias_init(sb, 0, m, h));
+      a[m] = mfma(sw[(kFwdBase[0] + m) * 64 + lane], b0,
+                  bias_init(sb, 0, m, h));
+    uint32_t bits[4];
     bf16x8 nb[4];
+#pragma unroll
     for (int layer = 1; layer < 4; ++layer) {
-      relu_operands(acc, nb);
-      for (int m = 0; m < 2; ++m) store_tile(act_ptr(act, layer), acc[m], m,
-                                             h, col, true);
+      bits[layer - 1] = relu_bits(a);
+      relu_operands(a, nb);
+      __bf16* al = act + (16 + (layer - 1) * kHid) * kRound;
+#pragma unroll
+      for (int m = 0; m < 2; ++m) put_tile(al, a[m], m, h, px, true);
 #pragma unroll
       for (int m = 0; m < 2; ++m) {
         f32x16 c = bias_init(sb, layer, m, h);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
-          c = mfma(wf[(kFwdBase[layer] + 4 * m + s) * 64 + lane], nb[s], c);
-        acc[m] = c;
+          c = mfma(sw[(kFwdBase[layer] + 4 * m + s) * 64 + lane], nb[s], c);
+        a[m] = c;
       }
     }
-    relu_operands(acc, nb);
-    for (int m = 0; m < 2; ++m) store_tile(act_ptr(act, 4), acc[m], m, h, col,
-                                           true);
+    bits[3] = relu_bits(a);
+    relu_operands(a, nb);
+    {
+      __bf16* a4 = act + (16 + 3 * kHid) * kRound;
+#pragma unroll
+      for (int m = 0; m < 2; ++m) put_tile(a4, a[m], m, h, px, true);
+    }
     f32x16 y = bias_init(sb, 4, 0, h);
 #pragma unroll
     for (int s = 0; s < 4; ++s)
-      y = mfma(wf[(kFwdBase[4] + s) * 64 + lane], nb[s], y);
+      y = mfma(sw[(kFwdBase[4] + s) * 64 + lane], nb[s], y);
     // ---- loss gradient (output units 0..2 live in lanes h == 0, regs 0..2)
     f32x16 d[2];
 #pragma unroll
@@ -362,84 +515,43 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
         d[0][r] = (((2.0f * e) * scale) * mk) * alb[r];
       }
     }
-    // ---- backward, layer 5 .. 1
-    for (int layer = 4; layer >= 0; --layer) {
-      const int mt = layer == 4 ? 1 : 2;
-      // bias gradient and delta to LDS
-      for (int m = 0; m < mt; ++m) {
+    if (h == 0) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int u = c_unit(m, r, h);
-          if (u < kOutSize[layer] && d[m][r] != 0.0f)
-            atomicAdd(&g[kB[layer] + u], d[m][r]);
-        }
-        store_tile(dlt, d[m], m, h, col, false);
-      }
-      lds_fence();
-      // dW[k_in][u_out] += sum over pixels a_{layer}[k_in] * delta[u_out]
-      const __bf16* ain = act_ptr(act, layer);
-      const int in_rows = layer == 0 ? 16 : 64;
-      const int nt = layer == 0 ? 1 : 2;
-      for (int m = 0; m < mt; ++m) {
-        const bf16x8 a0 = lds_operand(dlt, 32 * m + col, 32 * mt, 0, h);
-        const bf16x8 a1 = lds_operand(dlt, 32 * m + col, 32 * mt, 1, h);
-        for (int n = 0; n < nt; ++n) {
-          const int kin = 32 * n + col;
-          f32x16 c;
-#pragma unroll
-          for (int r = 0; r < 16; ++r) c[r] = 0.0f;
-          c = mfma(a0, lds_operand(ain, kin, in_rows, 0, h), c);
-          c = mfma(a1, lds_operand(ain, kin, in_rows, 1, h), c);
-          if (kin < kInSize[layer]) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-              const int u = c_unit(m, r, h);
-              if (u < kOutSize[layer] && c[r] != 0.0f)
-                atomicAdd(&g[kW[layer] + kin * kOutSize[layer] + u], c[r]);
-            }
-          }
-        }
-      }
-      if (layer == 0) break;
-      // d a_{layer} = W_layer d_out, then ReLU' from the stored activation.
-      bf16x8 db[4];
-      const int ks = layer == 4 ? 1 : 4;
-#pragma unroll
-      for (int m = 0; m < 2; ++m)
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-#pragma unroll
-          for (int j = 0; j < 8; ++j)
-            db[2 * m + s][j] = static_cast<__bf16>(d[m][8 * s + j]);
-      const __bf16* aprev = act_ptr(act, layer);
-      f32x16 nd[2];
-      for (int m = 0; m < 2; ++m) {
-        f32x16 c;
-#pragma unroll
-        for (int r = 0; r < 16; ++r) c[r] = 0.0f;
-        for (int s = 0; s < ks; ++s)
-          c = mfma(wb[(kBwdBase[layer] + (layer == 4 ? m : 4 * m + s)) * 64 +
-                      lane],
-                   db[s], c);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const float a = static_cast<float>(aprev[c_unit(m, r, h) * 32 + col]);
-          c[r] = a > 0.0f ? c[r] : 0.0f;
-        }
-        nd[m] = c;
-      }
-      d[0] = nd[0];
-      d[1] = nd[1];
-      lds_fence();  // dlt is rewritten by the next layer
+      for (int r = 0; r < 3; ++r)
+        dlt[swz(r, px)] = static_cast<__bf16>(d[0][r]);
     }
+    // ---- backward
+    f32x16 nd[2];
+    input_grad<4>(sw, d, bits[3], lane, nd);
+    backward_layer<4>(acc, bacc, act, dlt, d, nd, wave, h, col, px);
+    input_grad<3>(sw, d, bits[2], lane, nd);
+    backward_layer<3>(acc, bacc, act, dlt, d, nd, wave, h, col, px);
+    input_grad<2>(sw, d, bits[1], lane, nd);
+    backward_layer<2>(acc, bacc, act, dlt, d, nd, wave, h, col, px);
+    input_grad<1>(sw, d, bits[0], lane, nd);
+    backward_layer<1>(acc, bacc, act, dlt, d, nd, wave, h, col, px);
+    backward_layer<0>(acc, bacc, act, dlt, d, nd, wave, h, col, px);
   }
   for (int off = 32; off > 0; off >>= 1) partial += __shfl_xor(partial, off);
   if (lane == 0) wave_loss[wave] = partial;
-  __syncthreads();
-  // This block's gradient and loss, stored (reduced over blocks in a fixed
-  // order by reduce_kernel: no same-address atomics across blocks).
+  const int h = lane >> 5, col = lane & 31;
+  // This block's gradient and loss (every parameter has exactly one writer;
+  // reduce_kernel sums the blocks in a fixed order).
   float* out = partials + int64_t(blockIdx.x) * (kParams + 1);
-  for (int i = threadIdx.x; i < kParams; i += kBlock) out[i] = g[i];
+  write_tile<1>(out, acc[0], wave & 1, wave >> 1, h, col);
+  write_tile<2>(out, acc[1], wave & 1, wave >> 1, h, col);
+  write_tile<3>(out, acc[2], wave & 1, wave >> 1, h, col);
+  if (wave < 2) {
+    write_tile<0>(out, acc[3], wave, 0, h, col);
+    write_bias<0>(out, bacc[0], wave, h, col);
+    write_bias<1>(out, bacc[1], wave, h, col);
+    write_bias<2>(out, bacc[2], wave, h, col);
+    write_bias<3>(out, bacc[3], wave, h, col);
+  } else {
+    write_tile<4>(out, acc[3], 0, wave - 2, h, col);
+    if (wave == 2) write_bias<4>(out, bacc[4], 0, h, col);
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
     float t = 0.0f;
     for (int w = 0; w < kWaves; ++w) t += wave_loss[w];
@@ -474,13 +586,7 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
   p[i] = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-static_assert((kParams + 1 + 5 * 64) % 4 == 0, "LDS bf16 stage alignment");
-static_assert(((kActWave + kDltWave) * 2) % 16 == 0, "per-wave alignment");
-
-size_t train_lds_bytes() {
-  return sizeof(float) * (kParams + 1 + 5 * 64) +
-         sizeof(__bf16) * kWaves * (kActWave + kDltWave);
-}
+size_t train_lds_bytes() { return kTrainLdsBytes; }
 
 }  // namespace
 }  // namespace dr
@@ -548,9 +654,8 @@ int dr_mlp_train_step(const void* packed, const float* params, const float* x,
         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     lds_set = true;
   }
-  const int64_t chunks = (num_pixels + 31) / 32;
-  const int nb = int(std::min<int64_t>(nb0, (chunks + dr::kWaves - 1) /
-                                                dr::kWaves));
+  const int64_t rounds = (num_pixels + dr::kRound - 1) / dr::kRound;
+  const int nb = int(std::min<int64_t>(nb0, rounds));
   const auto* f = static_cast<const dr::bf16x8*>(packed);
   float* partials = static_cast<float*>(workspace);
   hipStream_t s = static_cast<hipStream_t>(stream);
