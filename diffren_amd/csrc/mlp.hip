@@ -180,9 +180,11 @@ __device__ __forceinline__ f32x16 bias_init(const float* sb, int layer, int m,
 }
 
 // ---------------------------------------------------------------------------
-// Inference: persistent waves; weights and biases in LDS (keeping the 30
-// operand blocks in registers would leave one wave per SIMD); the next
-// chunk's inputs are loaded while the current chunk runs on the matrix cores.
+// Inference: persistent waves, each on two 32-pixel chunks at a time (every
+// weight and bias read from LDS feeds two independent accumulator chains);
+// weights and biases in LDS (keeping the 30 operand blocks in registers would
+// leave one wave per SIMD); the next chunk pair's inputs are loaded while the
+// current pair runs on the matrix cores.
 
 __global__ __launch_bounds__(kBlock) void forward_kernel(
     const bf16x8* __restrict__ wf, const float* __restrict__ params,
@@ -198,47 +200,79 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(
   const bool modulate = (flags & DR_MLP_ALBEDO_MODULATE) != 0;
   const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
   const int64_t waves = int64_t(gridDim.x) * kWaves;
-  int64_t chunk = int64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6);
-  float raw[8], alb[3];
-  load_inputs(x, chunk * 32 + col, N, h, raw);
-  if (modulate) load_albedo(x, chunk * 32 + col, N, alb);
-  for (; chunk * 32 < N; chunk += waves) {
+  int64_t pair = int64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+  float raw[2][8], alb[2][3];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    load_inputs(x, (2 * pair + i) * 32 + col, N, h, raw[i]);
+    if (modulate) load_albedo(x, (2 * pair + i) * 32 + col, N, alb[i]);
+  }
+  for (; pair * 64 < N; pair += waves) {
     // Keeps the weight reads from LDS inside the loop (hoisting them would
     // pin all 30 operand blocks in registers again).
     __asm__ volatile("" ::: "memory");
-    const int64_t p = chunk * 32 + col;
-    const bf16x8 b0 = input_operand(raw, nrm, h);
-    float a[3] = {alb[0], alb[1], alb[2]};
-    // Prefetch the next chunk (its loads are in bounds or masked).
-    load_inputs(x, (chunk + waves) * 32 + col, N, h, raw);
-    if (modulate) load_albedo(x, (chunk + waves) * 32 + col, N, alb);
-    f32x16 acc[2];
+    bf16x8 b0[2];
+    float a[2][3];
 #pragma unroll
-    for (int m = 0; m < 2; ++m)
-      acc[m] = mfma(sw[(kFwdBase[0] + m) * 64 + lane], b0,
-                    bias_init(sb, 0, m, h));
-    bf16x8 nb[4];
+    for (int i = 0; i < 2; ++i) {
+      b0[i] = input_operand(raw[i], nrm, h);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a[i][c] = alb[i][c];
+    }
+    // Prefetch the next pair (its loads are in bounds or masked).
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int64_t q = (2 * (pair + waves) + i) * 32 + col;
+      load_inputs(x, q, N, h, raw[i]);
+      if (modulate) load_albedo(x, q, N, alb[i]);
+    }
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const f32x16 bias = bias_init(sb, 0, m, h);
+      const bf16x8 w = sw[(kFwdBase[0] + m) * 64 + lane];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) acc[i][m] = mfma(w, b0[i], bias);
+    }
+    bf16x8 nb[2][4];
 #pragma unroll
     for (int layer = 1; layer < 4; ++layer) {
-      relu_operands(acc, nb);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) relu_operands(acc[i], nb[i]);
 #pragma unroll
       for (int m = 0; m < 2; ++m) {
-        f32x16 c = bias_init(sb, layer, m, h);
+        f32x16 c[2];
+        c[0] = bias_init(sb, layer, m, h);
+        c[1] = c[0];
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
-          c = mfma(sw[(kFwdBase[layer] + 4 * m + s) * 64 + lane], nb[s], c);
-        acc[m] = c;
+        for (int s = 0; s < 4; ++s) {
+          const bf16x8 w = sw[(kFwdBase[layer] + 4 * m + s) * 64 + lane];
+#pragma unroll
+          for (int i = 0; i < 2; ++i) c[i] = mfma(w, nb[i][s], c[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i) acc[i][m] = c[i];
       }
     }
-    relu_operands(acc, nb);
-    f32x16 c = bias_init(sb, 4, 0, h);
 #pragma unroll
-    for (int s = 0; s < 4; ++s)
-      c = mfma(sw[(kFwdBase[4] + s) * 64 + lane], nb[s], c);
-    if (h == 0 && p < N) {
+    for (int i = 0; i < 2; ++i) relu_operands(acc[i], nb[i]);
+    f32x16 c[2];
+    c[0] = bias_init(sb, 4, 0, h);
+    c[1] = c[0];
 #pragma unroll
-      for (int r = 0; r < 3; ++r)
-        y[p * 3 + r] = modulate ? a[r] * c[r] : c[r];
+    for (int s = 0; s < 4; ++s) {
+      const bf16x8 w = sw[(kFwdBase[4] + s) * 64 + lane];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) c[i] = mfma(w, nb[i][s], c[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int64_t p = (2 * pair + i) * 32 + col;
+      if (h == 0 && p < N) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+          y[p * 3 + r] = modulate ? a[i][r] * c[i][r] : c[i][r];
+      }
     }
   }
 }
@@ -560,15 +594,26 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
 }
 
 // grad[i] += sum over blocks of partials[b][i]; *loss += sum of the losses.
+// A workgroup owns 64 consecutive entries; its four waves each sum every
+// fourth block row (64 consecutive floats per wave read), then the four
+// partial sums are added in a fixed order.
 __global__ __launch_bounds__(256) void reduce_kernel(
     const float* __restrict__ partials, int nblocks, float* __restrict__ grad,
     float* __restrict__ loss) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i > kParams) return;
+  __shared__ float part[4][64];
+  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + c;
   float t = 0.0f;
-  for (int b = 0; b < nblocks; ++b) t += partials[int64_t(b) * (kParams + 1) + i];
-  if (i < kParams) grad[i] += t;
-  else if (loss) *loss += t;
+  if (i <= kParams)
+    for (int b = w; b < nblocks; b += 4)
+      t += partials[int64_t(b) * (kParams + 1) + i];
+  part[w][c] = t;
+  __syncthreads();
+  if (w == 0 && i <= kParams) {
+    const float s = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
+    if (i < kParams) grad[i] += s;
+    else if (loss) *loss += s;
+  }
 }
 
 // Adam on float32 master weights (Kingma & Ba, 2015).
@@ -617,9 +662,9 @@ int dr_mlp_forward(const void* packed, const float* params, const float* x,
   DR_CHECK_ARG(num_pixels >= 0, "negative size");
   if (num_pixels == 0) return DR_OK;
   DR_CHECK_ARG(packed && params && x && y, "null pointer");
-  const int64_t chunks = (num_pixels + 31) / 32;
+  const int64_t pairs = (num_pixels + 63) / 64;
   int64_t nb = num_blocks > 0 ? num_blocks : 2048;
-  nb = std::min<int64_t>(nb, (chunks + dr::kWaves - 1) / dr::kWaves);
+  nb = std::min<int64_t>(nb, (pairs + dr::kWaves - 1) / dr::kWaves);
   dr::forward_kernel<<<unsigned(nb), dr::kBlock, 0,
                        static_cast<hipStream_t>(stream)>>>(
       static_cast<const dr::bf16x8*>(packed), params, x, num_pixels, y,
@@ -663,7 +708,7 @@ int dr_mlp_train_step(const void* packed, const float* params, const float* x,
       f, f + dr::kFwdBlocks * 64, params, x, target, mask, num_pixels,
       loss_scale, partials, input_norm, flags);
   DR_CHECK_LAUNCH("mlp train step");
-  dr::reduce_kernel<<<(dr::kParams + 1 + 255) / 256, 256, 0, s>>>(
+  dr::reduce_kernel<<<(dr::kParams + 1 + 63) / 64, 256, 0, s>>>(
       partials, nb, grad, loss);
   DR_CHECK_LAUNCH("mlp train reduce");
   return DR_OK;
