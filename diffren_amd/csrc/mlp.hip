@@ -183,8 +183,32 @@ __device__ __forceinline__ f32x16 bias_init(const float* sb, int layer, int m,
 // Inference: persistent waves, each on two 32-pixel chunks at a time (every
 // weight and bias read from LDS feeds two independent accumulator chains);
 // weights and biases in LDS (keeping the 30 operand blocks in registers would
-// leave one wave per SIMD); the next chunk pair's inputs are loaded while the
-// current pair runs on the matrix cores.
+// leave one wave per SIMD).  A chunk pair's 64 x 12 input floats are one
+// contiguous 3 KB span: they are loaded as three coalesced 16-byte loads per
+// lane (issued one pair ahead, while the current pair runs on the matrix
+// cores) and regrouped through a per-wave LDS stage into operand order.
+
+constexpr int kStage = 64 * kIn / 4;  // float4 per wave stage
+
+struct Fetch {
+  float4 v0, v1, v2;
+};
+
+__device__ __forceinline__ float4 fetch4(const float4* __restrict__ x4,
+                                         int64_t k, int64_t N) {
+  // float4 k lies inside pixel k / 3.
+  float4 t = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (k < 3 * N) t = x4[k];
+  return t;
+}
+
+__device__ __forceinline__ Fetch fetch_pair(const float4* __restrict__ x4,
+                                            int64_t pair, int64_t N,
+                                            int lane) {
+  const int64_t k = pair * kStage + lane;
+  return Fetch{fetch4(x4, k, N), fetch4(x4, k + 64, N),
+               fetch4(x4, k + 128, N)};
+}
 
 __global__ __launch_bounds__(kBlock) void forward_kernel(
     const bf16x8* __restrict__ wf, const float* __restrict__ params,
@@ -193,39 +217,43 @@ __global__ __launch_bounds__(kBlock) void forward_kernel(
   __shared__ float sb[5 * 64];
   __shared__ float nrm[2 * kIn];
   __shared__ bf16x8 sw[kFwdBlocks * 64];
+  __shared__ float4 stage_all[kWaves * kStage];
   stage_bias(params, sb);
   stage_norm(norm, nrm);
   for (int i = threadIdx.x; i < kFwdBlocks * 64; i += kBlock) sw[i] = wf[i];
   __syncthreads();
   const bool modulate = (flags & DR_MLP_ALBEDO_MODULATE) != 0;
   const int lane = threadIdx.x & 63, h = lane >> 5, col = lane & 31;
+  float4* stage = stage_all + (threadIdx.x >> 6) * kStage;
+  const float* stage_f = reinterpret_cast<const float*>(stage);
+  const float4* x4 = reinterpret_cast<const float4*>(x);
   const int64_t waves = int64_t(gridDim.x) * kWaves;
   int64_t pair = int64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6);
-  float raw[2][8], alb[2][3];
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    load_inputs(x, (2 * pair + i) * 32 + col, N, h, raw[i]);
-    if (modulate) load_albedo(x, (2 * pair + i) * 32 + col, N, alb[i]);
-  }
+  Fetch v = fetch_pair(x4, pair, N, lane);
   for (; pair * 64 < N; pair += waves) {
     // Keeps the weight reads from LDS inside the loop (hoisting them would
     // pin all 30 operand blocks in registers again).
     __asm__ volatile("" ::: "memory");
+    stage[lane] = v.v0;
+    stage[64 + lane] = v.v1;
+    stage[128 + lane] = v.v2;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     bf16x8 b0[2];
     float a[2][3];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      b0[i] = input_operand(raw[i], nrm, h);
+      const float* px = stage_f + (32 * i + col) * kIn;
+      float raw[8];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) a[i][c] = alb[i][c];
-    }
-    // Prefetch the next pair (its loads are in bounds or masked).
+      for (int j = 0; j < 8; ++j) raw[j] = 8 * h + j < kIn ? px[8 * h + j] : 0.0f;
+      b0[i] = input_operand(raw, nrm, h);
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int64_t q = (2 * (pair + waves) + i) * 32 + col;
-      load_inputs(x, q, N, h, raw[i]);
-      if (modulate) load_albedo(x, q, N, alb[i]);
+      for (int c = 0; c < 3; ++c) a[i][c] = px[9 + c];
     }
+    // The next pair's inputs (in bounds or zero) while this pair computes.
+    v = fetch_pair(x4, pair + waves, N, lane);
     f32x16 acc[2][2];
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
@@ -662,6 +690,8 @@ int dr_mlp_forward(const void* packed, const float* params, const float* x,
   DR_CHECK_ARG(num_pixels >= 0, "negative size");
   if (num_pixels == 0) return DR_OK;
   DR_CHECK_ARG(packed && params && x && y, "null pointer");
+  DR_CHECK_ARG((reinterpret_cast<uintptr_t>(x) & 15) == 0,
+               "features must be 16-byte aligned");
   const int64_t pairs = (num_pixels + 63) / 64;
   int64_t nb = num_blocks > 0 ? num_blocks : 2048;
   nb = std::min<int64_t>(nb, (pairs + dr::kWaves - 1) / dr::kWaves);
