@@ -80,11 +80,21 @@ SIGNATURES = {
                                                 _c_ptr]),
     "dr_transform_homogeneous_bwd": (_c_int, [_c_ptr, _c_ptr, _c_int, _c_i64,
                                               _c_ptr, _c_ptr]),
+    "dr_diff_barycentrics_fwd": (_c_int, [_c_ptr, _c_i64, _c_ptr, _c_int,
+                                          _c_ptr, _c_int, _c_int, _c_int,
+                                          _c_ptr, _c_ptr]),
+    "dr_diff_barycentrics_bwd": (_c_int, [_c_ptr, _c_i64, _c_ptr, _c_int,
+                                          _c_ptr, _c_int, _c_int, _c_int,
+                                          _c_ptr, _c_ptr, _c_ptr]),
     "dr_interpolate_fwd": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_int, _c_ptr,
                                     _c_int, _c_int, _c_ptr, _c_i64, _c_ptr,
                                     _c_ptr]),
     "dr_interpolate_bwd": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_int, _c_ptr,
                                     _c_int, _c_int, _c_i64, _c_ptr, _c_ptr]),
+    "dr_interpolate_bwd_barycentrics": (_c_int, [_c_ptr, _c_ptr, _c_int,
+                                                 _c_ptr, _c_int, _c_int,
+                                                 _c_ptr, _c_i64, _c_ptr,
+                                                 _c_ptr]),
     "dr_splat_coefficient": (ctypes.c_float, []),
     "dr_splat_fwd": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_int, _c_int, _c_int,
                               _c_int, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr]),

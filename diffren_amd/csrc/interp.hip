@@ -95,6 +95,37 @@ __global__ __launch_bounds__(kBlock) void interp_bwd_kernel(
   }
 }
 
+// d_barys[p, k] = sum_c dout[p, c] A[v_k, c] for foreground p, 0 elsewhere:
+// the barycentric half of the interpolation VJP, needed when the
+// barycentrics are themselves differentiable (rasterize.py:120-121).
+__global__ __launch_bounds__(kBlock) void interp_bwd_barys_kernel(
+    const int32_t* __restrict__ ids, const int32_t* __restrict__ tris,
+    int32_t T, const float* __restrict__ A, int32_t V, int32_t C,
+    const float* __restrict__ dout, int64_t N, float* __restrict__ dbarys) {
+  const int64_t p = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (p >= N) return;
+  int v0, v1, v2;
+  float* d = dbarys + 3 * p;
+  if (!pixel_vertices(ids, tris, T, V, p, v0, v1, v2)) {
+    d[0] = d[1] = d[2] = 0.0f;
+    return;
+  }
+  const float* g = dout + p * C;
+  const float* a0 = A + int64_t(v0) * C;
+  const float* a1 = A + int64_t(v1) * C;
+  const float* a2 = A + int64_t(v2) * C;
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+  for (int c = 0; c < C; ++c) {
+    const float gc = g[c];
+    s0 += gc * a0[c];
+    s1 += gc * a1[c];
+    s2 += gc * a2[c];
+  }
+  d[0] = s0;
+  d[1] = s1;
+  d[2] = s2;
+}
+
 // clip[v, i] = ((M[i,0] x + M[i,1] y) + M[i,2] z) + M[i,3] w  (w = 1 for xyz
 // input): transforms.py:40-78 for a [4, 4] matrix, one fixed summation order.
 __global__ __launch_bounds__(kBlock) void transform_kernel(
@@ -201,6 +232,29 @@ int dr_interpolate_bwd(const int32_t* triangle_ids, const float* barycentrics,
       triangle_ids, barycentrics, triangles, num_triangles, d_out,
       num_vertices, channels, num_pixels, d_attributes);
   DR_CHECK_LAUNCH("interpolate bwd");
+  return DR_OK;
+}
+
+int dr_interpolate_bwd_barycentrics(const int32_t* triangle_ids,
+                                    const int32_t* triangles,
+                                    int32_t num_triangles,
+                                    const float* attributes,
+                                    int32_t num_vertices, int32_t channels,
+                                    const float* d_out, int64_t num_pixels,
+                                    float* d_barycentrics,
+                                    dr_stream_t stream) {
+  DR_CHECK_ARG(channels > 0, "channels must be > 0");
+  DR_CHECK_ARG(num_pixels >= 0, "num_pixels must be >= 0");
+  DR_CHECK_ARG(num_triangles >= 0 && num_vertices >= 0, "negative mesh size");
+  if (num_pixels == 0) return DR_OK;
+  DR_CHECK_ARG(triangle_ids && d_out && d_barycentrics, "null pointer");
+  DR_CHECK_ARG(num_triangles == 0 || (triangles && attributes),
+               "null mesh pointer");
+  dr::interp_bwd_barys_kernel<<<dr::blocks_for(num_pixels), dr::kBlock, 0,
+                                static_cast<hipStream_t>(stream)>>>(
+      triangle_ids, triangles, num_triangles, attributes, num_vertices,
+      channels, d_out, num_pixels, d_barycentrics);
+  DR_CHECK_LAUNCH("interpolate bwd barycentrics");
   return DR_OK;
 }
 

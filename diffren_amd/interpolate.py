@@ -4,8 +4,10 @@ Mirrors diffren.jax.internal.interpolate (interpolate.py:24-63, 118-170):
 interpolate_vertex_attribute(attribute [V,C], framebuffer, background_value)
 -> [L,H,W,C], with a hand-written backward kernel (scatter-add of
 barycentric-weighted gradients to the three vertices of each pixel's
-triangle).  Gradients flow to the attribute only: barycentrics and ids come
-from the non-differentiable rasterizer (rasterize_triangles_xla.py:51-52).
+triangle).  Gradients flow to the attribute, and to the barycentrics when
+they are differentiable (rasterize_triangles(compute_diff_barys=True),
+rasterize.py:120-121); the rasterizer's own barycentrics and ids are
+constants (rasterize_triangles_xla.py:51-52).
 """
 import torch
 
@@ -27,22 +29,32 @@ class _Interpolate(torch.autograd.Function):
             triangles.shape[0], _native.ptr(a), a.shape[0], C,
             _native.ptr(background), n, _native.ptr(out),
             _native.stream_handle()), "dr_interpolate_fwd")
-    ctx.save_for_backward(ids, barys, triangles)
-    ctx.shape = a.shape
+    ctx.save_for_backward(ids, barys, triangles, a)
     return out
 
   @staticmethod
   def backward(ctx, d_out):
-    ids, barys, triangles = ctx.saved_tensors
-    V, C = ctx.shape
-    da = torch.zeros((V, C), dtype=torch.float32, device=d_out.device)
+    ids, barys, triangles, a = ctx.saved_tensors
+    V, C = a.shape
     d_out = d_out.to(torch.float32).contiguous()
-    _native.check(
-        _native.lib().dr_interpolate_bwd(
-            _native.ptr(ids), _native.ptr(barys), _native.ptr(triangles),
-            triangles.shape[0], _native.ptr(d_out), V, C, ids.numel(),
-            _native.ptr(da), _native.stream_handle()), "dr_interpolate_bwd")
-    return da, None, None, None, None
+    da = db = None
+    if ctx.needs_input_grad[0]:
+      da = torch.zeros((V, C), dtype=torch.float32, device=d_out.device)
+      _native.check(
+          _native.lib().dr_interpolate_bwd(
+              _native.ptr(ids), _native.ptr(barys), _native.ptr(triangles),
+              triangles.shape[0], _native.ptr(d_out), V, C, ids.numel(),
+              _native.ptr(da), _native.stream_handle()), "dr_interpolate_bwd")
+    if ctx.needs_input_grad[2]:
+      db = torch.empty(tuple(ids.shape) + (3,), dtype=torch.float32,
+                       device=d_out.device)
+      _native.check(
+          _native.lib().dr_interpolate_bwd_barycentrics(
+              _native.ptr(ids), _native.ptr(triangles), triangles.shape[0],
+              _native.ptr(a), V, C, _native.ptr(d_out), ids.numel(),
+              _native.ptr(db), _native.stream_handle()),
+          "dr_interpolate_bwd_barycentrics")
+    return da, None, db, None, None
 
 
 def _background(background_value, C, device):

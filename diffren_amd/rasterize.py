@@ -13,9 +13,16 @@ image at a time (the reference's vmap_method='sequential_unrolled', :68-75),
 `batched=True` rasterizes all images in one set of launches.
 
 `rasterize_triangles` mirrors diffren.jax.internal.rasterize.rasterize_triangles
-(diffren/jax/internal/rasterize.py:26-124).
+(diffren/jax/internal/rasterize.py:26-124), including compute_diff_barys=True:
+`differentiable_barycentrics` (:127-162) recomputes the barycentrics from the
+clip-space vertices on the GPU with a hand-written backward, so gradients
+reach the vertices through every attribute interpolated with them.
+`normalized_pixel_coordinates`, `compute_triangle_matrix` and
+`compute_barycentric_coordinates` (:165-256) are the same functions on torch
+tensors, for callers that use them directly.
 """
 import ctypes
+import dataclasses
 
 import torch
 
@@ -168,17 +175,13 @@ def rasterize_triangles_primitive(vertices, triangles, image_width,
 
 def rasterize_triangles(vertices, triangles, camera, image_width,
                         image_height, num_layers=1,
-                        face_culling_mode=constants.FaceCullingMode.BACK,
+                        face_culling_mode=constants.FaceCullingMode.NONE,
                         compute_diff_barys=False):
   """Rasterizes triangles into a Framebuffer (rasterize.py:26-124).
 
   vertices: [V, 3] or [V, 4] f32 (clip space if camera is None);
   triangles: [T, 3] i32; camera: [4, 4] or None.
   """
-  if compute_diff_barys:
-    raise NotImplementedError(
-        "compute_diff_barys is not supported; use the splat path for "
-        "gradients")
   if vertices.dim() != 2:
     raise ValueError(f"vertices must have shape [V, 3|4] but has shape "
                      f"{tuple(vertices.shape)}")
@@ -196,8 +199,99 @@ def rasterize_triangles(vertices, triangles, camera, image_width,
     if vertices.shape[-1] != 4:
       raise ValueError("vertices must be [V, 4] when camera is None")
     clip = vertices
-  ids, z, b = rasterize_triangles_primitive(clip, triangles, image_width,
+  tris = triangles.to(torch.int32)
+  ids, z, b = rasterize_triangles_primitive(clip.detach(), tris, image_width,
                                             image_height, num_layers,
                                             face_culling_mode)
-  return fb.Framebuffer(barycentrics=b, triangle_id=ids.unsqueeze(-1),
-                        triangles=triangles.to(torch.int32), z_buffer=z)
+  out = fb.Framebuffer(barycentrics=b, triangle_id=ids.unsqueeze(-1),
+                       triangles=tris, z_buffer=z)
+  if compute_diff_barys:
+    out = differentiable_barycentrics(out, clip, tris)
+  return out
+
+
+class _DiffBarycentrics(torch.autograd.Function):
+  """dr_diff_barycentrics_fwd / _bwd: barycentrics [L, H, W, 3] as a
+  differentiable function of the clip-space vertices [V, 4]."""
+
+  @staticmethod
+  def forward(ctx, clip, triangles, ids):
+    c = clip.detach().to(torch.float32).contiguous()
+    L, H, W = ids.shape
+    out = torch.empty((L, H, W, 3), dtype=torch.float32, device=c.device)
+    _native.check(
+        _native.lib().dr_diff_barycentrics_fwd(
+            _native.ptr(c), c.shape[0], _native.ptr(triangles),
+            triangles.shape[0], _native.ptr(ids), W, H, L, _native.ptr(out),
+            _native.stream_handle()), "dr_diff_barycentrics_fwd")
+    ctx.save_for_backward(c, triangles, ids)
+    return out
+
+  @staticmethod
+  def backward(ctx, d_barys):
+    c, triangles, ids = ctx.saved_tensors
+    L, H, W = ids.shape
+    d_clip = torch.zeros_like(c)
+    d_barys = d_barys.to(torch.float32).contiguous()
+    _native.check(
+        _native.lib().dr_diff_barycentrics_bwd(
+            _native.ptr(c), c.shape[0], _native.ptr(triangles),
+            triangles.shape[0], _native.ptr(ids), W, H, L,
+            _native.ptr(d_barys), _native.ptr(d_clip),
+            _native.stream_handle()), "dr_diff_barycentrics_bwd")
+    return d_clip, None, None
+
+
+def differentiable_barycentrics(framebuffer, clip_space_vertices, triangles):
+  """Framebuffer with barycentrics recomputed differentiably from the
+  clip-space vertices [V, 4] (rasterize.py:127-162); zero on background."""
+  if clip_space_vertices.dim() != 2 or clip_space_vertices.shape[-1] != 4:
+    raise ValueError("clip_space_vertices must be [V, 4]")
+  _native.require_device(clip_space_vertices)
+  ids = framebuffer.triangle_id[..., 0].contiguous()
+  if ids.dim() != 3:
+    raise ValueError("differentiable_barycentrics takes one [L, H, W] "
+                     "framebuffer")
+  tris = triangles.to(torch.int32).contiguous()
+  b = _DiffBarycentrics.apply(clip_space_vertices, tris, ids)
+  return dataclasses.replace(framebuffer, barycentrics=b)
+
+
+def normalized_pixel_coordinates(image_width, image_height, device=None):
+  """(x, y) pixel-centre coordinates [H, W] in [-1, 1] (rasterize.py:165-189):
+  (2i + 1) / size - 1 in float32."""
+  x = (2 * torch.arange(image_width, device=device) + 1).to(
+      torch.float32) / float(image_width) - 1
+  y = (2 * torch.arange(image_height, device=device) + 1).to(
+      torch.float32) / float(image_height) - 1
+  yy, xx = torch.meshgrid(y, x, indexing="ij")
+  return xx, yy
+
+
+def compute_triangle_matrix(clip_space_vertices, triangle):
+  """[3, 3] adjugate of the triangle's (x, y, w) columns times the sign of
+  its determinant (rasterize.py:192-225)."""
+  xyzw = clip_space_vertices[triangle.long(), :]
+  xyw = torch.cat((xyzw[:, :2], xyzw[:, 3:4]), dim=-1).T
+  d11 = xyw[1, 1] * xyw[2, 2] - xyw[1, 2] * xyw[2, 1]
+  d21 = xyw[1, 2] * xyw[2, 0] - xyw[1, 0] * xyw[2, 2]
+  d31 = xyw[1, 0] * xyw[2, 1] - xyw[1, 1] * xyw[2, 0]
+  d12 = xyw[2, 1] * xyw[0, 2] - xyw[2, 2] * xyw[0, 1]
+  d22 = xyw[2, 2] * xyw[0, 0] - xyw[2, 0] * xyw[0, 2]
+  d32 = xyw[2, 0] * xyw[0, 1] - xyw[2, 1] * xyw[0, 0]
+  d13 = xyw[0, 1] * xyw[1, 2] - xyw[0, 2] * xyw[1, 1]
+  d23 = xyw[0, 2] * xyw[1, 0] - xyw[0, 0] * xyw[1, 2]
+  d33 = xyw[0, 0] * xyw[1, 1] - xyw[0, 1] * xyw[1, 0]
+  matrix = torch.stack([torch.stack([d11, d12, d13]),
+                        torch.stack([d21, d22, d23]),
+                        torch.stack([d31, d32, d33])])
+  det = xyw[0, 0] * d11 + xyw[1, 0] * d12 + xyw[2, 0] * d13
+  sign = torch.sign(det) + (det == 0).to(det.dtype)
+  return sign * matrix
+
+
+def compute_barycentric_coordinates(triangle_matrix, px, py):
+  """[3] barycentrics of a point (rasterize.py:228-256), nan_to_num'ed."""
+  b = (triangle_matrix[:, 0] * px + triangle_matrix[:, 1] * py +
+       triangle_matrix[:, 2])
+  return torch.nan_to_num(b / torch.sum(b))

@@ -162,6 +162,37 @@ int dr_transform_homogeneous_bwd(const float* matrix, const float* d_clip,
                                  float* d_vertices, dr_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
+/* Differentiable barycentrics                                              */
+/* ------------------------------------------------------------------------ */
+
+/* barycentrics [L, H, W, 3] recomputed from the clip-space vertices
+ * [num_vertices, 4] (16-byte aligned) for the triangle id of every
+ * pixel-layer (triangle_ids [L, H, W] from the rasterizer): normalised
+ * 2D-homogeneous edge functions at pixel centres px = (2x+1)/W - 1,
+ * py = (2y+1)/H - 1, nan_to_num of r / sum(r), 0 on background.  Replaces
+ * differentiable_barycentrics, normalized_pixel_coordinates,
+ * compute_triangle_matrix and compute_barycentric_coordinates
+ * (diffren/jax/internal/rasterize.py:127-256), called from
+ * rasterize_triangles(compute_diff_barys=True) (:120-121). */
+int dr_diff_barycentrics_fwd(const float* clip_vertices, int64_t num_vertices,
+                             const int32_t* triangles, int32_t num_triangles,
+                             const int32_t* triangle_ids, int32_t width,
+                             int32_t height, int32_t num_layers,
+                             float* barycentrics, dr_stream_t stream);
+
+/* d_clip_vertices [num_vertices, 4] += the VJP of dr_diff_barycentrics_fwd
+ * for d_barycentrics [L, H, W, 3] (x, y, w columns; z gets none).  Float
+ * atomics: the summation order is not fixed; the caller clears the output.
+ * No reference counterpart (the reference uses XLA autodiff of
+ * rasterize.py:206-256). */
+int dr_diff_barycentrics_bwd(const float* clip_vertices, int64_t num_vertices,
+                             const int32_t* triangles, int32_t num_triangles,
+                             const int32_t* triangle_ids, int32_t width,
+                             int32_t height, int32_t num_layers,
+                             const float* d_barycentrics,
+                             float* d_clip_vertices, dr_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Interpolation                                                            */
 /* ------------------------------------------------------------------------ */
 
@@ -187,6 +218,21 @@ int dr_interpolate_bwd(const int32_t* triangle_ids, const float* barycentrics,
                        const float* d_out, int32_t num_vertices,
                        int32_t channels, int64_t num_pixels,
                        float* d_attributes, dr_stream_t stream);
+
+/* d_barycentrics[p, k] = sum_c d_out[p, c] attributes[v_k, c] for
+ * foreground p, 0 on background (written, not accumulated): the
+ * barycentric half of the interpolation VJP, used when the barycentrics
+ * come from dr_diff_barycentrics_fwd.  No reference counterpart (XLA
+ * autodiff of interpolate.py:49-50 through the differentiable barycentrics
+ * of rasterize.py:120-121). */
+int dr_interpolate_bwd_barycentrics(const int32_t* triangle_ids,
+                                    const int32_t* triangles,
+                                    int32_t num_triangles,
+                                    const float* attributes,
+                                    int32_t num_vertices, int32_t channels,
+                                    const float* d_out, int64_t num_pixels,
+                                    float* d_barycentrics,
+                                    dr_stream_t stream);
 
 /* ------------------------------------------------------------------------ */
 /* Splatting                                                                */

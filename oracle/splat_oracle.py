@@ -65,7 +65,9 @@ def interpolate(attribute, ids, barys, triangles, background=None):
   a = attribute
   ids = torch.as_tensor(np.asarray(ids)).long()
   tris = torch.as_tensor(np.asarray(triangles)).long()
-  b = torch.as_tensor(np.asarray(barys)).to(a.dtype)
+  # Tensors (possibly differentiable barycentrics) are used as they are.
+  b = (barys if isinstance(barys, torch.Tensor) else
+       torch.as_tensor(np.asarray(barys))).to(a.dtype)
   vid = tris[torch.where(ids < 0, ids + tris.shape[0], ids)]  # wraps like jnp
   corners = a[vid]                                            # [...,3,C]
   summed = (corners[..., 0, :] * b[..., 0:1] +

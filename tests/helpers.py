@@ -38,10 +38,15 @@ def images_are_near(baseline, result, max_outlier_fraction=0.005,
   return frac <= max_outlier_fraction, f"{frac:.5f} of pixels are outliers"
 
 
-def check_image(image, fixture, resize_to=None, exact=False):
+def check_image(image, fixture, resize_to=None, exact=False,
+                add_transparency=True):
   """test_utils.py:143-172 semantics; returns (ok, message)."""
   from PIL import Image
-  img = pil_formatted(image)
+  if add_transparency:
+    img = pil_formatted(image)
+  else:
+    img = np.clip(255.0 * np.asarray(image, np.float32), 0.0,
+                  255.0).astype(np.uint8)
   if resize_to:
     img = np.array(Image.fromarray(img).resize(resize_to,
                                                Image.Resampling.NEAREST))
