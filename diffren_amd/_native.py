@@ -20,6 +20,8 @@ _c_size = ctypes.c_size_t
 
 RASTER_SPARSE = 1
 MLP_ALBEDO_MODULATE = 1
+PAD_CONSTANT = 0
+PAD_EDGE = 1
 
 
 class RasterConfig(ctypes.Structure):
@@ -102,6 +104,12 @@ SIGNATURES = {
     "dr_splat_bwd": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_int, _c_int, _c_int,
                               _c_int, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_size,
                               _c_ptr]),
+    "dr_bilinear_resample_fwd": (_c_int, [_c_ptr, _c_int, _c_int, _c_int,
+                                          _c_ptr, _c_i64, _c_int, _c_ptr,
+                                          _c_ptr]),
+    "dr_bilinear_resample_bwd": (_c_int, [_c_ptr, _c_int, _c_int, _c_int,
+                                          _c_ptr, _c_i64, _c_int, _c_ptr,
+                                          _c_ptr, _c_ptr, _c_ptr]),
     "dr_gbuffer": (_c_int, [_c_ptr, _c_ptr, _c_ptr, _c_int, _c_ptr, _c_ptr,
                             _c_ptr, _c_int, _c_ptr, _c_i64, _c_ptr, _c_ptr]),
     "dr_phong_shade": (_c_int, [_c_ptr, _c_i64, _c_ptr, _c_int, _c_ptr,

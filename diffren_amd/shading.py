@@ -4,6 +4,9 @@ MLP shader with its training loop.
 gbuffer(...)          -> [L,H,W,12] position, normal, view direction, albedo
 phong(...)            -> [...,3] many-light form of shaders.point_light
                          (diffren/jax/utils/shaders.py:100-173), ambient once
+texture_map(...)      -> [...,H,W,C] shaders.texture_map
+                         (diffren/jax/utils/shaders.py:47-72) on the
+                         bilinear resampling kernel, edge clamped
 MLPShader             12 -> 64x4 -> 3 MLP, bfloat16 MFMA, float32 accumulation,
                       trained by the library's own forward/backward/Adam
                       kernels; `shade_with_fallback` keeps the Phong result
@@ -15,6 +18,7 @@ import numpy as np
 import torch
 
 from diffren_amd import _native
+from diffren_amd import image
 
 PHONG_DEFAULTS = dict(ka=0.3, kd=1.4, ks=0.5, ns=16.0,
                       attenuation_constant=1.0, attenuation_linear=0.7,
@@ -40,6 +44,25 @@ def gbuffer(framebuffer, positions, normals, albedo, eye):
           _native.ptr(eye_t), ids.numel(), _native.ptr(out),
           _native.stream_handle()), "dr_gbuffer")
   return out
+
+
+def texture_map(uv_image, texture_image):
+  """Bilinear texture lookup with CLAMP_TO_EDGE (shaders.py:47-72).
+
+  uv_image: [..., H, W, 2] UV coordinates in [0, 1]; texture_image: a
+  [Ht, Wt, C] float tensor (the reference also accepts a file name; decode
+  it with PIL and pass the array).  Returns [..., H, W, C], differentiable
+  with respect to both inputs.
+  """
+  if isinstance(texture_image, str):
+    raise ValueError("texture_map takes a decoded [Ht, Wt, C] texture "
+                     "tensor")
+  tex = torch.as_tensor(texture_image, dtype=torch.float32,
+                        device=uv_image.device)
+  scale = torch.tensor([tex.shape[1], tex.shape[0]], dtype=torch.float32,
+                       device=uv_image.device)
+  query = uv_image * scale
+  return image.bilinear_resample(tex, query, pad_mode="edge")
 
 
 def phong(gbuf, lights, **params):

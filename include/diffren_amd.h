@@ -344,6 +344,37 @@ int dr_render_splat_backward(const dr_splat_job* jobs_device,
 
 
 /* ------------------------------------------------------------------------ */
+/* Texture sampling                                                         */
+/* ------------------------------------------------------------------------ */
+
+enum { DR_PAD_CONSTANT = 0, DR_PAD_EDGE = 1 };
+
+/* out [num_samples, C] = bilinear samples of image [H, W, C] (HWC float32)
+ * at locations [num_samples, 2] (x, y in image units, 8-byte aligned; texel
+ * (i, j) is centred at (j + 0.5, i + 0.5)), outside the image padded with 0
+ * (DR_PAD_CONSTANT) or the edge texel (DR_PAD_EDGE).  Replaces
+ * image.bilinear_resample (diffren/jax/utils/image.py:21-118) for
+ * image_order = locations_order = 'HWC'; shaders.texture_map
+ * (diffren/jax/utils/shaders.py:47-72) is DR_PAD_EDGE at uv * (W, H). */
+int dr_bilinear_resample_fwd(const float* image, int32_t height,
+                             int32_t width, int32_t channels,
+                             const float* locations, int64_t num_samples,
+                             int32_t pad_mode, float* out,
+                             dr_stream_t stream);
+
+/* VJP of dr_bilinear_resample_fwd for d_out [num_samples, C]:
+ * d_image [H, W, C] += (float atomics; NULL to skip) and d_locations
+ * [num_samples, 2] = (written; NULL to skip).  The weights' clip to [0, 1]
+ * is differentiated as JAX differentiates minimum(maximum(t, 0), 1): 1/2 at
+ * a bound.  No reference counterpart (XLA autodiff of image.py:52-92). */
+int dr_bilinear_resample_bwd(const float* image, int32_t height,
+                             int32_t width, int32_t channels,
+                             const float* locations, int64_t num_samples,
+                             int32_t pad_mode, const float* d_out,
+                             float* d_image, float* d_locations,
+                             dr_stream_t stream);
+
+/* ------------------------------------------------------------------------ */
 /* Deferred shading: G-buffer, many-light Phong, learned MLP shader         */
 /* ------------------------------------------------------------------------ */
 
