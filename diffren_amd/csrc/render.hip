@@ -419,22 +419,36 @@ struct Record {
 //  * each wave covers a 16x4 patch of pixels, whose fragments reference each
 //    mesh vertex ~4 times on average (neighbouring pixels' triangles share
 //    vertices);
-//  * the wave stages its fragments' three vertex records in LDS, finds
-//    repeated vertices with a small LDS hash table, adds each repeat into
-//    the first occurrence's record, and flushes only first occurrences;
+//  * the wave finds repeated vertices among its fragments' three vertex
+//    records with a small LDS hash table, keeps one LDS row per distinct
+//    vertex (the first occurrence writes it, repeats add into it) and
+//    flushes only those rows;
 //  * the flush puts R consecutive lanes on one record, because grouped
 //    atomics run ~7x faster than one scattered address per lane.
 constexpr int kHashSlots = 256;   // > 3 x 64 entries per wave and layer
+// Distinct vertices a wave keeps in LDS per layer (its 192 entries reference
+// about a quarter as many; further distinct vertices are flushed directly).
+constexpr int kRows = 96;
+
+template <int C>
+__device__ __forceinline__ void flush_record(float* grads, int v,
+                                             const float* rec) {
+  constexpr int R = Record<C>::kSize;
+  if (grads == nullptr) return;
+#pragma unroll
+  for (int c = 0; c < R; ++c)
+    if (rec[c] != 0.0f) unsafeAtomicAdd(grads + int64_t(v) * R + c, rec[c]);
+}
 
 template <int C>
 __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     const dr_splat_job* __restrict__ jobs, int num_jobs, int L, float coef) {
   constexpr int R = Record<C>::kSize;
-  constexpr int kPerWave = 3 * 64;             // entries per wave and layer
-  __shared__ float stage_val[kBX * kBY / 64][kPerWave][R];
-  __shared__ int stage_vtx[kBX * kBY / 64][kPerWave];
+  __shared__ float row_val[kBX * kBY / 64][kRows][R];
+  __shared__ int row_vtx[kBX * kBY / 64][kRows];
+  __shared__ int row_count[kBX * kBY / 64];
   __shared__ int hash_key[kBX * kBY / 64][kHashSlots];
-  __shared__ int hash_own[kBX * kBY / 64][kHashSlots];
+  __shared__ int hash_row[kBX * kBY / 64][kHashSlots];
   const int64_t tile = blockIdx.x;
   const dr_splat_job& J = jobs[find_job(jobs, num_jobs, tile)];
   const int W = J.width, H = J.height;
@@ -445,10 +459,11 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   // Wave w covers columns 16w .. 16w+15 of the 64x4 tile, all 4 rows.
   const int x = (local % ntx) * kBX + 16 * wave + (lane & 15);
   const int y = (local / ntx) * kBY + (lane >> 4);
-  float(*val)[R] = stage_val[wave];
-  int* vtx = stage_vtx[wave];
+  float(*val)[R] = row_val[wave];
+  int* vtx = row_vtx[wave];
+  int* nrows = &row_count[wave];
   int* hkey = hash_key[wave];
-  int* hown = hash_own[wave];
+  int* hrow = hash_row[wave];
   const bool inside = x < W && y < H;
   const int64_t P = int64_t(W) * H;
   const int64_t p = int64_t(y) * W + x;
@@ -547,24 +562,17 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         for (int c = 4 + C; c < R; ++c) rec[i][c] = 0.0f;
       }
     }
-    // Wave-local append of 3 entries per fragment, dedupe, grouped flush.
+    // Wave-local dedupe into compact LDS rows (one per distinct vertex, in
+    // the order first occurrences claim them), then a grouped flush.
     const uint64_t act = __ballot(frag);
     if (act == 0) continue;
     for (int i = lane; i < kHashSlots; i += 64) hkey[i] = -1;
-    int base = 0;
-    int slot[3] = {0, 0, 0};
-    bool first[3] = {false, false, false};
-    if (frag) {
-      base = 3 * __popcll(act & ((uint64_t(1) << lane) - 1));
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        vtx[base + i] = k.v[i];
-#pragma unroll
-        for (int c = 0; c < R; ++c) val[base + i][c] = rec[i][c];
-      }
-    }
+    if (lane == 0) *nrows = 0;
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     __builtin_amdgcn_wave_barrier();
+    float* grads = J.d_records;
+    int slot[3] = {0, 0, 0};
+    bool first[3] = {false, false, false};
     if (frag) {
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
@@ -580,8 +588,18 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         slot[i] = int(h);
       }
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
-        if (first[i]) hown[slot[i]] = base + i;
+      for (int i = 0; i < 3; ++i) {
+        if (!first[i]) continue;
+        const int r = atomicAdd(nrows, 1);
+        hrow[slot[i]] = r < kRows ? r : -1;
+        if (r < kRows) {
+          vtx[r] = k.v[i];
+#pragma unroll
+          for (int c = 0; c < R; ++c) val[r][c] = rec[i][c];
+        } else {
+          flush_record<C>(grads, k.v[i], rec[i]);
+        }
+      }
     }
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     __builtin_amdgcn_wave_barrier();
@@ -589,25 +607,25 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         if (first[i]) continue;
-        const int o = hown[slot[i]];
+        const int r = hrow[slot[i]];
+        if (r < 0) {
+          flush_record<C>(grads, k.v[i], rec[i]);
+          continue;
+        }
 #pragma unroll
         for (int c = 0; c < R; ++c)
-          if (rec[i][c] != 0.0f) atomicAdd(&val[o][c], rec[i][c]);
-        vtx[base + i] = -1;
+          if (rec[i][c] != 0.0f) atomicAdd(&val[r][c], rec[i][c]);
       }
     }
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     __builtin_amdgcn_wave_barrier();
-    const int n = 3 * __popcll(act);
-    float* grads = J.d_records;
+    const int n = min(*nrows, kRows);
     if (grads) {
       for (int e0 = 0; e0 < n; e0 += 64 / R) {
         const int e = e0 + lane / R, c = lane % R;
         if (e < n) {
-          const int vv = vtx[e];
           const float g = val[e][c];
-          if (vv >= 0 && g != 0.0f)
-            unsafeAtomicAdd(grads + int64_t(vv) * R + c, g);
+          if (g != 0.0f) unsafeAtomicAdd(grads + int64_t(vtx[e]) * R + c, g);
         }
       }
     }
