@@ -169,20 +169,24 @@ __device__ __forceinline__ float norm_scale(float w) {
   return 1.0f / (fmaxf(w - 1.0f, 0.0f) + 1.0f);
 }
 
-// LDS stage, structure of arrays over [L][kRegion]: pixel-centre offsets
-// x, y, viewport z, colour[C].  Background / outside-the-image entries have
-// x = NaN as the mask (the fg mask is 1 exactly where the rasterizer stored a
-// triangle).
+// LDS: for all layers, the region's triangle ids st[L][kRegion] and viewport
+// z sz[L][kRegion] (the surface indices need every layer's z at once); then,
+// one layer at a time, pixel-centre offsets x, y and colour[C] of that layer
+// (x = NaN marks background / outside the image: the fg mask is 1 exactly
+// where the rasterizer stored a triangle).  Staging a single layer's splats
+// keeps the block's LDS small enough for seven resident blocks per CU.
 template <int C>
 __global__ __launch_bounds__(kBX* kBY) void target_kernel(
     const dr_splat_job* __restrict__ jobs, int num_jobs, int L, float coef,
     int grad_mode, float loss_scale, float* __restrict__ loss_partials) {
   extern __shared__ float lds[];
   const int R = kRegion;
-  float* sx = lds;
-  float* sy = sx + L * R;
-  float* sz = sy + L * R;
-  float* sc = sz + L * R;  // [L][R][C]
+  int* st = reinterpret_cast<int*>(lds);
+  float* sz = lds + L * R;
+  float* sx = sz + L * R;
+  float* sy = sx + R;
+  float* sc = sy + R;  // [R][C]
+  __shared__ int layer_live[kMaxLayers];
 
   const int64_t tile = blockIdx.x;
   const dr_splat_job& J = jobs[find_job(jobs, num_jobs, tile)];
@@ -192,84 +196,35 @@ __global__ __launch_bounds__(kBX* kBY) void target_kernel(
   const int x0 = (local % ntx) * kBX, y0 = (local / ntx) * kBY;
   const int64_t P = int64_t(W) * H;
   const int tid = threadIdx.y * kBX + threadIdx.x;
+  constexpr int kThreads = kBX * kBY;
 
-  // Each thread stages kChunk items at a time, issuing the loads of all of
-  // them stage by stage (ids; then triangles, barycentrics, z; then vertex
-  // gathers) so that kChunk dependent load chains are in flight at once.
-  constexpr int kChunk = 4;
-  for (int base = tid; base < L * R; base += kChunk * kBX * kBY) {
-    int tri[kChunk];
-    int64_t qs[kChunk];
-#pragma unroll
-    for (int j = 0; j < kChunk; ++j) {
-      const int i = base + j * kBX * kBY;
-      tri[j] = -1;
-      qs[j] = 0;
-      if (i < L * R) {
-        const int l = i / R, r = i % R;
-        const int px = x0 - 1 + r % kRX, py = y0 - 1 + r / kRX;
-        if (px >= 0 && px < W && py >= 0 && py < H) {
-          qs[j] = l * P + int64_t(py) * W + px;
-          tri[j] = J.triangle_ids[qs[j]];
-        }
-      }
+  // Ids and z of every layer (z buffer read only under a triangle).
+  if (tid < kMaxLayers) layer_live[tid] = 0;
+  __syncthreads();
+  for (int i = tid; i < L * R; i += kThreads) {
+    const int l = i / R, r = i % R;
+    const int px = x0 - 1 + r % kRX, py = y0 - 1 + r / kRX;
+    int t = -1;
+    float z = 1.0f;
+    if (px >= 0 && px < W && py >= 0 && py < H) {
+      const int64_t q = l * P + int64_t(py) * W + px;
+      t = J.triangle_ids[q];
+      if (t >= 0 && t < J.num_triangles) z = viewport_z(J.z_buffer[q]);
+      else t = -1;
     }
-    Corner k[kChunk];
-    float zb[kChunk];
-#pragma unroll
-    for (int j = 0; j < kChunk; ++j) {
-      const int t = tri[j];
-      zb[j] = 1.0f;
-      if (t >= 0 && t < J.num_triangles) {
-#pragma unroll
-        for (int e = 0; e < 3; ++e) {
-          k[j].v[e] = J.triangles[3 * int64_t(t) + e];
-          k[j].b[e] = J.barycentrics[3 * qs[j] + e];
-        }
-        zb[j] = J.z_buffer[qs[j]];
-      } else {
-        tri[j] = -1;
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < kChunk; ++j) {
-      const int i = base + j * kBX * kBY;
-      if (i >= L * R) break;
-      float x = __int_as_float(0x7fc00000), y = 0.0f, z = 1.0f;
-      float col[C];
-#pragma unroll
-      for (int c = 0; c < C; ++c) col[c] = 0.0f;
-      bool ok = tri[j] >= 0;
-#pragma unroll
-      for (int e = 0; e < 3; ++e)
-        ok = ok && unsigned(k[j].v[e]) < unsigned(J.num_vertices);
-      if (ok) {
-        z = viewport_z(zb[j]);
-        float cx, cy, cw;
-        splat_position(J, k[j], x, y, cx, cy, cw);
-        const float* a0 = J.colors + int64_t(k[j].v[0]) * C;
-        const float* a1 = J.colors + int64_t(k[j].v[1]) * C;
-        const float* a2 = J.colors + int64_t(k[j].v[2]) * C;
-#pragma unroll
-        for (int c = 0; c < C; ++c)
-          col[c] = interp3(k[j], a0[c], a1[c], a2[c]);
-      }
-      sx[i] = x;
-      sy[i] = y;
-      sz[i] = z;
-#pragma unroll
-      for (int c = 0; c < C; ++c) sc[i * C + c] = col[c];
-    }
+    st[i] = t;
+    sz[i] = z;
+    if (t >= 0) layer_live[l] = 1;
   }
   __syncthreads();
 
   const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
-  float partial = 0.0f;
-  if (x < W && y < H) {
-    const int rT = (threadIdx.y + 1) * kRX + threadIdx.x + 1;
+  const bool in_image = x < W && y < H;
+  const int rT = (threadIdx.y + 1) * kRX + threadIdx.x + 1;
+  int surf[9];
+  if (in_image) {
     float zT[kMaxLayers], zS[kMaxLayers];
     for (int l = 0; l < L; ++l) zT[l] = sz[l * R + rT];
-    int surf[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
       const int u = t / 3 - 1, v = t % 3 - 1;
@@ -277,33 +232,94 @@ __global__ __launch_bounds__(kBX* kBY) void target_kernel(
       for (int l = 0; l < L; ++l) zS[l] = sz[l * R + rS];
       surf[t] = surface_index(zS, zT, L);
     }
-    float acc[3][C + 1];
+  }
+  float acc[3][C + 1];
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
+  for (int k = 0; k < 3; ++k)
 #pragma unroll
-      for (int c = 0; c <= C; ++c) acc[k][c] = 0.0f;
-    for (int l = 0; l < L; ++l) {
+    for (int c = 0; c <= C; ++c) acc[k][c] = 0.0f;
+
+  for (int l = 0; l < L; ++l) {
+    if (!layer_live[l]) continue;  // block-uniform
+    // Stage layer l's splats; each thread issues the loads of its items
+    // stage by stage (triangles + barycentrics, then vertex gathers) so its
+    // dependent load chains are in flight together.
+    constexpr int kChunk = (kRegion + kThreads - 1) / kThreads;
+    Corner k[kChunk];
+    bool ok[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      const int i = tid + j * kThreads;
+      ok[j] = false;
+      if (i < R) {
+        const int t = st[l * R + i];
+        if (t >= 0) {
+          const int px = x0 - 1 + i % kRX, py = y0 - 1 + i / kRX;
+          const int64_t q = l * P + int64_t(py) * W + px;
+          ok[j] = true;
+#pragma unroll
+          for (int e = 0; e < 3; ++e) {
+            k[j].v[e] = J.triangles[3 * int64_t(t) + e];
+            k[j].b[e] = J.barycentrics[3 * q + e];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j) {
+      const int i = tid + j * kThreads;
+      if (i >= R) break;
+      float px = __int_as_float(0x7fc00000), py = 0.0f;
+      float col[C];
+#pragma unroll
+      for (int c = 0; c < C; ++c) col[c] = 0.0f;
+      bool good = ok[j];
+#pragma unroll
+      for (int e = 0; e < 3; ++e)
+        good = good && unsigned(k[j].v[e]) < unsigned(J.num_vertices);
+      if (good) {
+        float cx, cy, cw;
+        splat_position(J, k[j], px, py, cx, cy, cw);
+        const float* a0 = J.colors + int64_t(k[j].v[0]) * C;
+        const float* a1 = J.colors + int64_t(k[j].v[1]) * C;
+        const float* a2 = J.colors + int64_t(k[j].v[2]) * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+          col[c] = interp3(k[j], a0[c], a1[c], a2[c]);
+      }
+      sx[i] = px;
+      sy[i] = py;
+#pragma unroll
+      for (int c = 0; c < C; ++c) sc[i * C + c] = col[c];
+    }
+    __syncthreads();
+    if (in_image) {
 #pragma unroll
       for (int t = 0; t < 9; ++t) {
         const int u = t / 3 - 1, v = t % 3 - 1;
-        const int i = l * R + rT - v * kRX - u;
+        const int i = rT - v * kRX - u;
         const float px = sx[i];
         if (px != px) continue;  // background or outside the image
         float dx, dy;
         const float w = tap_weight(coef, px, sy[i], u, v, dx, dy);
-        const int k = layer_class(surf[t], l);
+        const int kc = layer_class(surf[t], l);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           const float wc = w * sc[i * C + c];
-          acc[0][c] += k == 0 ? wc : 0.0f;
-          acc[1][c] += k == 1 ? wc : 0.0f;
-          acc[2][c] += k == 2 ? wc : 0.0f;
+          acc[0][c] += kc == 0 ? wc : 0.0f;
+          acc[1][c] += kc == 1 ? wc : 0.0f;
+          acc[2][c] += kc == 2 ? wc : 0.0f;
         }
-        acc[0][C] += k == 0 ? w : 0.0f;
-        acc[1][C] += k == 1 ? w : 0.0f;
-        acc[2][C] += k == 2 ? w : 0.0f;
+        acc[0][C] += kc == 0 ? w : 0.0f;
+        acc[1][C] += kc == 1 ? w : 0.0f;
+        acc[2][C] += kc == 2 ? w : 0.0f;
       }
     }
+    __syncthreads();  // the next layer's staging overwrites sx, sy, sc
+  }
+
+  float partial = 0.0f;
+  if (in_image) {
     float s[3], n[3][C];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -635,7 +651,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
 }
 
 size_t target_lds_bytes(int L, int C) {
-  return size_t(L) * kRegion * (3 + C) * sizeof(float);
+  return (size_t(L) * 2 + 2 + C) * kRegion * sizeof(float);
 }
 
 float splat_coef() {
