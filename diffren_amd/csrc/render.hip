@@ -175,8 +175,10 @@ __device__ __forceinline__ float norm_scale(float w) {
 // (x = NaN marks background / outside the image: the fg mask is 1 exactly
 // where the rasterizer stored a triangle).  Staging a single layer's splats
 // keeps the block's LDS small enough for seven resident blocks per CU.
+// Six resident waves per SIMD fit without spilling for up to 4 channels.
 template <int C>
-__global__ __launch_bounds__(kBX* kBY) void target_kernel(
+__global__ __launch_bounds__(kBX* kBY)
+__attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
     const dr_splat_job* __restrict__ jobs, int num_jobs, int L, float coef,
     int grad_mode, float loss_scale, float* __restrict__ loss_partials) {
   extern __shared__ float lds[];
@@ -221,7 +223,9 @@ __global__ __launch_bounds__(kBX* kBY) void target_kernel(
   const int x = x0 + threadIdx.x, y = y0 + threadIdx.y;
   const bool in_image = x < W && y < H;
   const int rT = (threadIdx.y + 1) * kRX + threadIdx.x + 1;
-  int surf[9];
+  // The 9 surface indices, one signed byte each (tap t in byte t % 4 of
+  // word t / 4): the same bytes the gradient scratch records.
+  uint32_t sw[3] = {0u, 0u, 0u};
   if (in_image) {
     float zT[kMaxLayers], zS[kMaxLayers];
     for (int l = 0; l < L; ++l) zT[l] = sz[l * R + rT];
@@ -230,7 +234,8 @@ __global__ __launch_bounds__(kBX* kBY) void target_kernel(
       const int u = t / 3 - 1, v = t % 3 - 1;
       const int rS = rT - v * kRX - u;
       for (int l = 0; l < L; ++l) zS[l] = sz[l * R + rS];
-      surf[t] = surface_index(zS, zT, L);
+      sw[t >> 2] |= (uint32_t(surface_index(zS, zT, L)) & 0xffu)
+                    << (8 * (t & 3));
     }
   }
   float acc[3][C + 1];
@@ -302,7 +307,8 @@ __global__ __launch_bounds__(kBX* kBY) void target_kernel(
         if (px != px) continue;  // background or outside the image
         float dx, dy;
         const float w = tap_weight(coef, px, sy[i], u, v, dx, dy);
-        const int kc = layer_class(surf[t], l);
+        const int kc = layer_class(
+            int(int8_t(sw[t >> 2] >> (8 * (t & 3)))), l);
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           const float wc = w * sc[i * C + c];
@@ -382,12 +388,8 @@ __global__ __launch_bounds__(kBX* kBY) void target_kernel(
           *reinterpret_cast<float4*>(gp + k * GS + c) =
               make_float4(rec[c], rec[c + 1], rec[c + 2], rec[c + 3]);
       }
-      uint32_t sw[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-      for (int t = 0; t < 9; ++t)
-        sw[t >> 2] |= (uint32_t(surf[t]) & 0xffu) << (8 * (t & 3));
       *reinterpret_cast<uint4*>(gp + 3 * GS) =
-          make_uint4(sw[0], sw[1], sw[2], sw[3]);
+          make_uint4(sw[0], sw[1], sw[2], 0u);
     }
   }
   if (grad_mode == 1) {
