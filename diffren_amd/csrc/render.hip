@@ -493,17 +493,17 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   // Surface indices of this source with its 9 targets, as the target pass
   // computed them: byte t of each target's scratch record.
   constexpr int GS = Scratch<C>::kStride, GR = Scratch<C>::kRecord;
-  int surf[9];
+  // One signed byte per tap, tap t in byte t % 4 of word t / 4.
+  uint32_t surf[3] = {0u, 0u, 0u};
   if (any) {
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
       const int u = t / 3 - 1, v = t % 3 - 1;
       const int tx = x + u, ty = y + v;
-      surf[t] = 0;
       if (tx < 0 || tx >= W || ty < 0 || ty >= H) continue;
-      const int8_t* sb = reinterpret_cast<const int8_t*>(
+      const uint8_t* sb = reinterpret_cast<const uint8_t*>(
           J.grad_scratch + (int64_t(ty) * W + tx) * GR + 3 * GS);
-      surf[t] = sb[t];
+      surf[t >> 2] |= uint32_t(sb[t]) << (8 * (t & 3));
     }
   }
   // Every lane of the wave runs every layer iteration (ballots below).
@@ -533,7 +533,8 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         if (tx < 0 || tx >= W || ty < 0 || ty >= H) continue;
         float dx, dy;
         const float w = tap_weight(coef, sx, sy, u, v, dx, dy);
-        const int cls = layer_class(surf[t], l);
+        const int cls =
+            layer_class(int(int8_t(surf[t >> 2] >> (8 * (t & 3)))), l);
         const float* gp =
             J.grad_scratch + (int64_t(ty) * W + tx) * GR + cls * GS;
         float gv[GS];
