@@ -548,13 +548,14 @@ __device__ __forceinline__ void reference_insert(float* lz, int* lid, int L,
   }
 }
 
-// Block-cooperative; every thread of the block must call it.
-__device__ void replay_pixel(const dr_raster_job& job, int ix, int iy,
-                             float sx2, float sy2, int cull, bool overflow,
-                             uint32_t cnt, uint64_t off,
-                             const uint32_t* __restrict__ bins, int L,
-                             uint64_t* slot, int tp, uint64_t* cand,
-                             int* cand_n) {
+// Block-cooperative; every thread of the block must call it.  Inlined: as a
+// called function its register demand (145 VGPRs) became the whole tile
+// kernel's, three resident waves per SIMD instead of five.
+__device__ __forceinline__ void replay_pixel(
+    const dr_raster_job& job, int ix, int iy, float sx2, float sy2, int cull,
+    bool overflow, uint32_t cnt, uint64_t off,
+    const uint32_t* __restrict__ bins, int L, uint64_t* slot, int tp,
+    uint64_t* cand, int* cand_n) {
   const uint64_t last = slot[(L - 1) * tp];
   const uint32_t zmax = last == kEmpty ? 0xffffffffu : uint32_t(last >> 32);
   const int64_t items = overflow ? int64_t(job.num_triangles) : int64_t(cnt);
