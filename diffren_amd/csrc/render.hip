@@ -458,12 +458,32 @@ __device__ __forceinline__ void flush_record(float* grads, int v,
     if (rec[c] != 0.0f) unsafeAtomicAdd(grads + int64_t(v) * R + c, rec[c]);
 }
 
+// The wave's 16x4 sources splat onto an 18x6 patch of targets.  Their
+// gradient-scratch records are staged in LDS once, coalesced, before any
+// dependent load: the 9 surface bytes of every record, and (for C <= 4) the
+// "surface" class of G, which nearly every layer-0 fragment reads; other
+// classes and layers read G from global memory.  The stage is dead once
+// layer 0 has been differentiated, so the per-vertex rows reuse its space.
+constexpr int kPatchX = 18, kPatchY = 6, kPatch = kPatchX * kPatchY;
+
+template <int C>
+struct SourceLds {
+  static constexpr int R = Record<C>::kSize;
+  static constexpr int GS = Scratch<C>::kStride;
+  static constexpr bool kStageG = GS <= 8;
+  static constexpr int kStageBytes =
+      kPatch * 16 + (kStageG ? kPatch * GS * 4 : 0);
+  static constexpr int kRowBytes = kRows * R * 4 + kRows * 4;
+  static constexpr int kBytes =
+      ((kStageBytes > kRowBytes ? kStageBytes : kRowBytes) + 15) & ~15;
+};
+
 template <int C>
 __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     const dr_splat_job* __restrict__ jobs, int num_jobs, int L, float coef) {
   constexpr int R = Record<C>::kSize;
-  __shared__ float row_val[kBX * kBY / 64][kRows][R];
-  __shared__ int row_vtx[kBX * kBY / 64][kRows];
+  using S = SourceLds<C>;
+  __shared__ float4 wave_lds[kBX * kBY / 64][S::kBytes / 16];
   __shared__ int row_count[kBX * kBY / 64];
   __shared__ int hash_key[kBX * kBY / 64][kHashSlots];
   __shared__ int hash_row[kBX * kBY / 64][kHashSlots];
@@ -477,8 +497,10 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   // Wave w covers columns 16w .. 16w+15 of the 64x4 tile, all 4 rows.
   const int x = (local % ntx) * kBX + 16 * wave + (lane & 15);
   const int y = (local / ntx) * kBY + (lane >> 4);
-  float(*val)[R] = row_val[wave];
-  int* vtx = row_vtx[wave];
+  float(*val)[R] = reinterpret_cast<float(*)[R]>(wave_lds[wave]);
+  int* vtx = reinterpret_cast<int*>(wave_lds[wave]) + kRows * R;
+  const uint4* stage_surf = reinterpret_cast<const uint4*>(wave_lds[wave]);
+  const float4* stage_g = wave_lds[wave] + kPatch;
   int* nrows = &row_count[wave];
   int* hkey = hash_key[wave];
   int* hrow = hash_row[wave];
@@ -486,24 +508,57 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   const int64_t P = int64_t(W) * H;
   const int64_t p = int64_t(y) * W + x;
 
+  constexpr int GS = Scratch<C>::kStride, GR = Scratch<C>::kRecord;
+  // Stage the patch's records (out-of-image targets as zeros).  Records of
+  // targets that received no splat weight were never written; they are
+  // only read for taps of foreground sources, whose targets all were.
+  {
+    const int px0 = x - (lane & 15) - 1, py0 = y - (lane >> 4) - 1;
+    uint4* ss = reinterpret_cast<uint4*>(wave_lds[wave]);
+    float4* sg = wave_lds[wave] + kPatch;
+    for (int e = lane; e < kPatch; e += 64) {
+      const int tx = px0 + e % kPatchX, ty = py0 + e / kPatchX;
+      uint4 sv = make_uint4(0u, 0u, 0u, 0u);
+      float4 g4[S::kStageG ? GS / 4 : 1];
+#pragma unroll
+      for (int i = 0; i < (S::kStageG ? GS / 4 : 1); ++i)
+        g4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (tx >= 0 && tx < W && ty >= 0 && ty < H) {
+        const float* gp = J.grad_scratch + (int64_t(ty) * W + tx) * GR;
+        sv = *reinterpret_cast<const uint4*>(gp + 3 * GS);
+        if (S::kStageG) {
+#pragma unroll
+          for (int i = 0; i < GS / 4; ++i)
+            g4[i] = *reinterpret_cast<const float4*>(gp + GS + 4 * i);
+        }
+      }
+      ss[e] = sv;
+      if (S::kStageG) {
+#pragma unroll
+        for (int i = 0; i < GS / 4; ++i) sg[e * (GS / 4) + i] = g4[i];
+      }
+    }
+  }
   bool any = false;
   if (inside)
     for (int l = 0; l < L; ++l) any |= J.triangle_ids[l * P + p] >= 0;
+  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
 
   // Surface indices of this source with its 9 targets, as the target pass
-  // computed them: byte t of each target's scratch record.
-  constexpr int GS = Scratch<C>::kStride, GR = Scratch<C>::kRecord;
-  // One signed byte per tap, tap t in byte t % 4 of word t / 4.
+  // computed them: byte t of each target's record.  One signed byte per
+  // tap, tap t in byte t % 4 of word t / 4.
+  auto patch_index = [&](int t) {
+    const int u = t / 3 - 1, v = t % 3 - 1;
+    return (1 + (lane >> 4) + v) * kPatchX + 1 + (lane & 15) + u;
+  };
   uint32_t surf[3] = {0u, 0u, 0u};
   if (any) {
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
-      const int u = t / 3 - 1, v = t % 3 - 1;
-      const int tx = x + u, ty = y + v;
-      if (tx < 0 || tx >= W || ty < 0 || ty >= H) continue;
-      const uint8_t* sb = reinterpret_cast<const uint8_t*>(
-          J.grad_scratch + (int64_t(ty) * W + tx) * GR + 3 * GS);
-      surf[t >> 2] |= uint32_t(sb[t]) << (8 * (t & 3));
+      const uint4 sv = stage_surf[patch_index(t)];
+      const uint32_t word = t < 4 ? sv.x : (t < 8 ? sv.y : sv.z);
+      surf[t >> 2] |= ((word >> (8 * (t & 3))) & 0xffu) << (8 * (t & 3));
     }
   }
   // Every lane of the wave runs every layer iteration (ballots below).
@@ -535,13 +590,24 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         const float w = tap_weight(coef, sx, sy, u, v, dx, dy);
         const int cls =
             layer_class(int(int8_t(surf[t >> 2] >> (8 * (t & 3)))), l);
-        const float* gp =
-            J.grad_scratch + (int64_t(ty) * W + tx) * GR + cls * GS;
         float gv[GS];
+        if (S::kStageG && l == 0 && cls == 1) {
+          const float4* gs = stage_g + patch_index(t) * (GS / 4);
 #pragma unroll
-        for (int c = 0; c < GS; c += 4) {
-          const float4 q4 = *reinterpret_cast<const float4*>(gp + c);
-          gv[c] = q4.x; gv[c + 1] = q4.y; gv[c + 2] = q4.z; gv[c + 3] = q4.w;
+          for (int c = 0; c < GS; c += 4) {
+            const float4 q4 = gs[c / 4];
+            gv[c] = q4.x; gv[c + 1] = q4.y; gv[c + 2] = q4.z;
+            gv[c + 3] = q4.w;
+          }
+        } else {
+          const float* gp =
+              J.grad_scratch + (int64_t(ty) * W + tx) * GR + cls * GS;
+#pragma unroll
+          for (int c = 0; c < GS; c += 4) {
+            const float4 q4 = *reinterpret_cast<const float4*>(gp + c);
+            gv[c] = q4.x; gv[c + 1] = q4.y; gv[c + 2] = q4.z;
+            gv[c + 3] = q4.w;
+          }
         }
         float dw = gv[C];
 #pragma unroll
