@@ -150,3 +150,46 @@ def test_invalid_arguments_raise(device):
     rasterize.rasterize_triangles_primitive(vt, tt, 4, 4, 9, 0)
   with pytest.raises(ValueError):
     rasterize.rasterize_triangles_primitive(vt, tt, 4, 4, 1, 3)
+
+
+def _stack(num, rng, coplanar):
+  """`num` overlapping random triangles around the image centre; coplanar
+  ones all lie at z/w = 0.25 (every fragment ties), others at random z."""
+  v = []
+  for i in range(num):
+    c = rng.uniform(-0.3, 0.3, 2)
+    z = 0.25 if coplanar else rng.uniform(-0.9, 0.9)
+    w = rng.uniform(0.5, 2.0)
+    for k in range(3):
+      a = 2 * np.pi * (k / 3.0 + rng.uniform(-0.1, 0.1))
+      p = c + rng.uniform(0.3, 0.8) * np.array([np.cos(a), np.sin(a)])
+      v.append([p[0] * w, p[1] * w, z * w, w])
+  t = np.arange(3 * num, dtype=np.int32).reshape(num, 3)
+  return np.array(v, np.float32), t
+
+
+@pytest.mark.parametrize("layers", [1, 2, 4, 8])
+def test_exact_depth_ties_follow_reference_order(layers, device):
+  # Every fragment has the same z: the kept ids and their order are those of
+  # the reference's id-ordered insertion with strict '<', not simply the
+  # lowest ids.
+  rng = np.random.default_rng(layers)
+  v, t = _stack(40, rng, coplanar=True)
+  for cull in (0, 1):
+    assert_same(gpu_raster(v, t, 48, 40, layers, cull, device),
+                raster_oracle.rasterize(v, t, 48, 40, layers, cull))
+
+
+def test_eight_layer_stack(device):
+  rng = np.random.default_rng(11)
+  v, t = _stack(60, rng, coplanar=False)
+  got = gpu_raster(v, t, 64, 64, 8, 0, device)
+  assert_same(got, raster_oracle.rasterize(v, t, 64, 64, 8, 0))
+  assert (got[0][7] >= 0).any()  # the deepest layer is populated
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (2048, 160), (160, 2048), (31, 1)])
+def test_extreme_image_shapes(w, h, device):
+  v, t = golden_scenes.adversarial_scene(900 + w, 50, 200)
+  assert_same(gpu_raster(v, t, w, h, 3, 0, device),
+              raster_oracle.rasterize(v, t, w, h, 3, 0))
