@@ -339,26 +339,53 @@ __device__ __forceinline__ int swz(int row, int px) {
   return row * kRound + ((((px >> 3) ^ (row & 15)) << 3) | (px & 7));
 }
 
+// The lane-dependent parts of every swz() address of one iteration, so that
+// a tile write or operand read is a register plus a compile-time offset:
+//  * C register r of tile m sits in row 32m + 8(r>>2) + 4h + (r&3), whose
+//    low 4 bits are 4h | c with c = 8((r>>2)&1) + (r&3), disjoint from 4h:
+//    w[j] holds 128 * 4h + the swizzled column for c = c(j);
+//  * operand rows are 32 * tile + col, low 4 bits col & 15: r[s] holds
+//    128 col + the swizzled 8-pixel group of k-step s.
+struct LaneOfs {
+  int w[8];
+  int r[kRound / 16];
+};
+
+__device__ __forceinline__ LaneOfs lane_offsets(int h, int col, int px) {
+  LaneOfs o;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = 8 * (j >> 2) + (j & 3);
+    o.w[j] = 512 * h + ((((px >> 3) ^ (4 * h) ^ c) << 3) | (px & 7));
+  }
+#pragma unroll
+  for (int s = 0; s < kRound / 16; ++s)
+    o.r[s] = col * kRound + (((2 * s + h) ^ (col & 15)) << 3);
+  return o;
+}
+
 __device__ __forceinline__ void put_tile(__bf16* arr, const f32x16& c, int m,
-                                         int h, int px, bool relu) {
+                                         const LaneOfs& o, bool relu) {
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const float v = relu ? fmaxf(c[r], 0.0f) : c[r];
-    arr[swz(c_unit(m, r, h), px)] = static_cast<__bf16>(v);
+    arr[kRound * (32 * m + 8 * (r >> 2) + (r & 3)) +
+        o.w[4 * ((r >> 2) & 1) + (r & 3)]] = static_cast<__bf16>(v);
   }
 }
 
-// Operand with K = pixels 16s..16s+15 of the round: lane row, 8 pixels.
-__device__ __forceinline__ bf16x8 px_operand(const __bf16* arr, int row,
-                                             int nrows, int s, int h) {
-  if (row >= nrows) {
+// Operand with K = pixels 16s..16s+15 of the round: lane row 32 tile + col,
+// 8 pixels (zero beyond nrows).
+__device__ __forceinline__ bf16x8 px_operand(const __bf16* arr, int tile,
+                                             int nrows, int s,
+                                             const LaneOfs& o, int col) {
+  if (32 * tile + col >= nrows) {
     bf16x8 z;
 #pragma unroll
     for (int j = 0; j < 8; ++j) z[j] = static_cast<__bf16>(0.0f);
     return z;
   }
-  return *reinterpret_cast<const bf16x8*>(
-      arr + row * kRound + (((2 * s + h) ^ (row & 15)) << 3));
+  return *reinterpret_cast<const bf16x8*>(arr + 32 * tile * kRound + o.r[s]);
 }
 
 __device__ __forceinline__ float row_sum(const bf16x8& a, float acc) {
@@ -387,13 +414,14 @@ template <int L>
 __device__ __forceinline__ void weight_grad(f32x16& acc, float& bacc,
                                             const __bf16* dlt,
                                             const __bf16* ain, int m, int n,
-                                            int h, int col, bool bias) {
+                                            const LaneOfs& o, int col,
+                                            bool bias) {
   constexpr int out_rows = kOutSize[L];
   constexpr int in_rows = L == 0 ? 16 : kHid;
 #pragma unroll
   for (int s = 0; s < kRound / 16; ++s) {
-    const bf16x8 a = px_operand(dlt, 32 * m + col, out_rows, s, h);
-    const bf16x8 b = px_operand(ain, 32 * n + col, in_rows, s, h);
+    const bf16x8 a = px_operand(dlt, m, out_rows, s, o, col);
+    const bf16x8 b = px_operand(ain, n, in_rows, s, o, col);
     acc = mfma(a, b, acc);
     if (bias) bacc = row_sum(a, bacc);
   }
@@ -438,25 +466,25 @@ __device__ __forceinline__ void input_grad(const bf16x8* sw, const f32x16 d[2],
 template <int L>
 __device__ __forceinline__ void backward_layer(
     f32x16 acc[4], float bacc[5], __bf16* act, __bf16* dlt, f32x16 d[2],
-    const f32x16 nd[2], int wave, int h, int col, int px) {
+    const f32x16 nd[2], int wave, const LaneOfs& o, int col) {
   __syncthreads();
   const __bf16* ain = L == 0 ? act : act + (16 + (L - 1) * kHid) * kRound;
   const __bf16* dl = dlt + (L & 1) * kHid * kRound;
   if (L >= 1 && L <= 3) {
-    weight_grad<L>(acc[L - 1], bacc[L], dl, ain, wave & 1, wave >> 1, h, col,
+    weight_grad<L>(acc[L - 1], bacc[L], dl, ain, wave & 1, wave >> 1, o, col,
                    (wave >> 1) == 0);
   } else if (L == 0) {
-    if (wave < 2) weight_grad<0>(acc[3], bacc[0], dl, ain, wave, 0, h, col,
+    if (wave < 2) weight_grad<0>(acc[3], bacc[0], dl, ain, wave, 0, o, col,
                                  true);
   } else {
-    if (wave >= 2) weight_grad<4>(acc[3], bacc[4], dl, ain, 0, wave - 2, h,
+    if (wave >= 2) weight_grad<4>(acc[3], bacc[4], dl, ain, 0, wave - 2, o,
                                   col, wave == 2);
   }
   if (L > 0) {
     __bf16* next = dlt + ((L - 1) & 1) * kHid * kRound;
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-      put_tile(next, nd[m], m, h, px, false);
+      put_tile(next, nd[m], m, o, false);
       d[m] = nd[m];
     }
   }
@@ -521,6 +549,7 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
     __asm__ volatile("" : "+v"(h), "+v"(col));
     const int px = 32 * wave + col;  // pixel within the round
     const int64_t p = q * kRound + px;
+    const LaneOfs o = lane_offsets(h, col, px);
     // ---- forward, keeping layer inputs in LDS and ReLU signs in bits
     float raw[8];
     load_inputs(x, p, N, h, raw);
@@ -540,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
       relu_operands(a, nb);
       __bf16* al = act + (16 + (layer - 1) * kHid) * kRound;
 #pragma unroll
-      for (int m = 0; m < 2; ++m) put_tile(al, a[m], m, h, px, true);
+      for (int m = 0; m < 2; ++m) put_tile(al, a[m], m, o, true);
 #pragma unroll
       for (int m = 0; m < 2; ++m) {
         f32x16 c = bias_init(sb, layer, m, h);
@@ -555,7 +584,7 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
     {
       __bf16* a4 = act + (16 + 3 * kHid) * kRound;
 #pragma unroll
-      for (int m = 0; m < 2; ++m) put_tile(a4, a[m], m, h, px, true);
+      for (int m = 0; m < 2; ++m) put_tile(a4, a[m], m, o, true);
     }
     f32x16 y = bias_init(sb, 4, 0, h);
 #pragma unroll
@@ -585,14 +614,14 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
     // ---- backward
     f32x16 nd[2];
     input_grad<4>(sw, d, bits[3], lane, nd);
-    backward_layer<4>(acc, bacc, act, dlt, d, nd, wave, h, col, px);
+    backward_layer<4>(acc, bacc, act, dlt, d, nd, wave, o, col);
     input_grad<3>(sw, d, bits[2], lane, nd);
-    backward_layer<3>(acc, bacc, act, dlt, d, nd, wave, h, col, px);
+    backward_layer<3>(acc, bacc, act, dlt, d, nd, wave, o, col);
     input_grad<2>(sw, d, bits[1], lane, nd);
-    backward_layer<2>(acc, bacc, act, dlt, d, nd, wave, h, col, px);
+    backward_layer<2>(acc, bacc, act, dlt, d, nd, wave, o, col);
     input_grad<1>(sw, d, bits[0], lane, nd);
-    backward_layer<1>(acc, bacc, act, dlt, d, nd, wave, h, col, px);
-    backward_layer<0>(acc, bacc, act, dlt, d, nd, wave, h, col, px);
+    backward_layer<1>(acc, bacc, act, dlt, d, nd, wave, o, col);
+    backward_layer<0>(acc, bacc, act, dlt, d, nd, wave, o, col);
   }
   for (int off = 32; off > 0; off >>= 1) partial += __shfl_xor(partial, off);
   if (lane == 0) wave_loss[wave] = partial;
