@@ -389,9 +389,8 @@ def main():
                   algorithmic_bytes_per_launch=int(launch_bytes),
                   avg_launch_ms=round(launch_ms, 4), launches_per_step=ng)
 
-  # Plain path: one render_triangles per view + autograd.
-  for _ in range(1):
-    plain_step(w, args)
+  # Plain path: one render_triangles per view + autograd (one warm-up step).
+  plain_step(w, args)
   torch.cuda.synchronize()
   t0 = time.perf_counter()
   for _ in range(args.plain_steps):

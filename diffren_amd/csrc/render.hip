@@ -567,8 +567,6 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     const bool frag = any && corners(J, l * P + p, k);
     float rec[3][R];
     if (frag) {
-      const int64_t q = l * P + p;
-      (void)q;
       float sx, sy, cx, cy, cw;
       splat_position(J, k, sx, sy, cx, cy, cw);
       const float* a0 = J.colors + int64_t(k.v[0]) * C;
