@@ -60,8 +60,25 @@ class BatchedSplatRenderer:
     self.device = dev
     C = meshes[0]["colors"].shape[1]
     for m in meshes:
-      if m["colors"].shape[1] != C:
+      v, t, c = m["vertices"], m["triangles"], m["colors"]
+      _native.require_device(v, t, c)
+      if (v.dtype != torch.float32 or v.dim() != 2 or v.shape[1] != 3 or
+          not v.is_contiguous()):
+        raise ValueError("mesh vertices must be contiguous float32 [V, 3]")
+      if (t.dtype != torch.int32 or t.dim() != 2 or t.shape[1] != 3 or
+          not t.is_contiguous()):
+        raise ValueError("mesh triangles must be contiguous int32 [T, 3]")
+      if (c.dtype != torch.float32 or c.dim() != 2 or
+          c.shape[0] != v.shape[0] or not c.is_contiguous()):
+        raise ValueError("mesh colors must be contiguous float32 [V, C]")
+      if c.shape[1] != C:
         raise ValueError("all meshes need the same channel count")
+    for mi, cam, w, h in jobs:
+      if not 0 <= mi < len(meshes):
+        raise ValueError(f"job references mesh {mi} of {len(meshes)}")
+      if tuple(cam.shape) != (4, 4) or w <= 0 or h <= 0:
+        raise ValueError("jobs are (mesh index, camera [4, 4], width, "
+                         "height) with positive sizes")
     self.C = C
     # Per-mesh gradient records [V, R]: d position in cols 0..2, d colour in
     # cols 4..4+C-1 (one contiguous record per vertex for grouped atomics).

@@ -175,3 +175,15 @@ def test_vertex_gradient_matches_central_differences(device):
                                  1.0 / (R * R * 4))
   print(f"finite-difference relative error {err:.3e}")
   assert err <= 2e-2
+
+
+def test_invalid_batches_raise(device):
+  meshes = _meshes(device)
+  jobs = _jobs(meshes)
+  bad = dict(meshes[0], vertices=meshes[0]["vertices"].double())
+  with pytest.raises(ValueError):
+    batch.BatchedSplatRenderer([bad], jobs[:1], 1)
+  with pytest.raises(ValueError):
+    batch.BatchedSplatRenderer(meshes, [(5, jobs[0][1], 16, 16)], 1)
+  with pytest.raises(ValueError):
+    batch.BatchedSplatRenderer(meshes, [(0, jobs[0][1], 0, 16)], 1)
