@@ -48,6 +48,9 @@ def parse():
   p.add_argument("--no-graph", action="store_true")
   p.add_argument("--streams", type=int, default=4,
                  help="concurrent job groups in the fused step")
+  p.add_argument("--fused-transform", action="store_true",
+                 help="transform vertices inside the raster and splat loads "
+                 "instead of a transform launch into clip-space buffers")
   p.add_argument("--pmc-probe", action="store_true",
                  help="run only the fused step + a calibration copy, for "
                  "rocprofv3 --pmc passes")
@@ -141,7 +144,8 @@ def pmc_probe(args):
   R, L = args.resolution, args.layers
   jobs = [(0, w["cams_t"][i], R, R) for i in range(args.views)]
   r = batch.BatchedSplatRenderer([w["mesh"]], jobs, L, tile=(16, 16),
-                                 streams=args.streams)
+                                 streams=args.streams,
+                                 fused_transform=args.fused_transform)
   for t in r.targets:
     t.uniform_()
   for _ in range(3):
@@ -315,7 +319,8 @@ def main():
   del tr
 
   r = batch.BatchedSplatRenderer([w["mesh"]], jobs, L, tile="auto",
-                                 streams=args.streams)
+                                 streams=args.streams,
+                                 fused_transform=args.fused_transform)
   log(f"autotuned raster tile {r.raster_config.tile_w}x"
       f"{r.raster_config.tile_h}, bin capacity {r.raster_config.bin_capacity}"
       f", timings {r.autotune_timings}")
@@ -428,6 +433,8 @@ def main():
       "config": {"workload": "1M-triangle sphere, 1024x1024, 3 layers, 8 "
                              "views, RGBA, fwd + L2 loss + bwd, batched "
                              f"fused path on {args.streams} stream(s)" +
+                             (", transform on load" if args.fused_transform
+                              else "") +
                              (" in a HIP graph" if use_graph else ""),
                  "triangles": T, "vertices": V, "resolution": R, "layers": L,
                  "views": B, "channels": C, "parallelism": "single GPU"},

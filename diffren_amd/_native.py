@@ -33,7 +33,8 @@ class RasterConfig(ctypes.Structure):
 class RasterJob(ctypes.Structure):
   _fields_ = [("vertices", _c_ptr), ("triangles", _c_ptr),
               ("triangle_ids", _c_ptr), ("z_buffer", _c_ptr),
-              ("barycentrics", _c_ptr), ("num_vertices", _c_int),
+              ("barycentrics", _c_ptr), ("camera", _c_ptr),
+              ("num_vertices", _c_int),
               ("num_triangles", _c_int), ("width", _c_int),
               ("height", _c_int), ("triangle_begin", _c_i64),
               ("tile_begin", _c_i64)]
@@ -48,7 +49,8 @@ class PhongParams(ctypes.Structure):
 
 
 class SplatJob(ctypes.Structure):
-  _fields_ = [("clip_vertices", _c_ptr), ("triangles", _c_ptr),
+  _fields_ = [("clip_vertices", _c_ptr), ("positions", _c_ptr),
+              ("triangles", _c_ptr),
               ("colors", _c_ptr), ("camera", _c_ptr),
               ("triangle_ids", _c_ptr), ("z_buffer", _c_ptr),
               ("barycentrics", _c_ptr), ("image", _c_ptr),

@@ -43,7 +43,12 @@ class BatchedSplatRenderer:
            Cameras are copied at construction.
   tile:    raster tile (w, h), None for the default or "auto" for the
            autotuner (diffren_amd.autotune).
-  streams: number of job groups run concurrently (1 or 2 are sensible).
+  streams: number of job groups run concurrently (each on its own stream).
+  fused_transform: when True no clip-space buffers are written: the
+           rasterizer and the splat passes read the world-space vertices
+           and transform them by the job's camera as they load them (same
+           float expression as the transform launch, so results are
+           identical).
 
   After step(): images[i], loss (sum over jobs of loss_scale * |img - tgt|^2),
   d_vertices[m] [V,3] and d_colors[m] [V,C] (views of records[m]).  ids[i]
@@ -51,7 +56,8 @@ class BatchedSplatRenderer:
   """
 
   def __init__(self, meshes, jobs, num_layers, tile=None, bin_capacity=0,
-               loss_scale=None, autotune_cache=None, streams=1):
+               loss_scale=None, autotune_cache=None, streams=1,
+               fused_transform=False):
     lib = _native.lib()
     self.meshes = meshes
     self.jobs = jobs
@@ -80,6 +86,7 @@ class BatchedSplatRenderer:
         raise ValueError("jobs are (mesh index, camera [4, 4], width, "
                          "height) with positive sizes")
     self.C = C
+    self.fused_transform = bool(fused_transform)
     # Per-mesh gradient records [V, R]: d position in cols 0..2, d colour in
     # cols 4..4+C-1 (one contiguous record per vertex for grouped atomics).
     R = lib.dr_render_record_size(C)
@@ -99,13 +106,15 @@ class BatchedSplatRenderer:
     for i, (mi, _, _, _) in enumerate(jobs):
       self.mesh_views.setdefault(mi, []).append(i)
     self.clip_blocks, self.camera_blocks = {}, {}
-    for mi, views in self.mesh_views.items():
+    for mi, views in ([] if self.fused_transform else
+                      self.mesh_views.items()):
       V = meshes[mi]["vertices"].shape[0]
       self.clip_blocks[mi] = torch.empty((len(views), V, 4),
                                          dtype=torch.float32, device=dev)
       self.camera_blocks[mi] = torch.stack(
           [jobs[i][1].to(torch.float32).reshape(4, 4) for i in views])
-    self.clip = [self.clip_blocks[mi][self.mesh_views[mi].index(i)]
+    self.clip = [None if self.fused_transform else
+                 self.clip_blocks[mi][self.mesh_views[mi].index(i)]
                  for i, (mi, _, _, _) in enumerate(jobs)]
     sf = lib.dr_render_scratch_floats(C)
     npix = 0
@@ -161,7 +170,11 @@ class BatchedSplatRenderer:
     for k, i in enumerate(indices):
       mi, _, w, h = self.jobs[i]
       m = self.meshes[mi]
-      table[k].vertices = self.clip[i].data_ptr()
+      if self.fused_transform:
+        table[k].vertices = m["vertices"].data_ptr()
+        table[k].camera = self.cameras[i].data_ptr()
+      else:
+        table[k].vertices = self.clip[i].data_ptr()
       table[k].triangles = m["triangles"].data_ptr()
       table[k].triangle_ids = self.ids[i].data_ptr()
       table[k].z_buffer = self.z[i].data_ptr()
@@ -208,7 +221,10 @@ class BatchedSplatRenderer:
       mi, _, w, h = self.jobs[i]
       m = self.meshes[mi]
       j = table[k]
-      j.clip_vertices = self.clip[i].data_ptr()
+      if self.fused_transform:
+        j.positions = m["vertices"].data_ptr()
+      else:
+        j.clip_vertices = self.clip[i].data_ptr()
       j.triangles = m["triangles"].data_ptr()
       j.colors = m["colors"].data_ptr()
       j.camera = self.cameras[i].data_ptr()
@@ -236,6 +252,8 @@ class BatchedSplatRenderer:
   # -- stages ----------------------------------------------------------------
 
   def transform(self):
+    if self.fused_transform:
+      return
     lib = _native.lib()
     s = _native.stream_handle()
     for mi, views in self.mesh_views.items():

@@ -271,16 +271,37 @@ __global__ void store_job_kernel(dr_raster_job job,
   *slot = job;
 }
 
-// Loads triangle `t` of `job`; false if an index is out of range.
+// Clip-space position of world point (x, y, z, 1) under the row-major 4x4
+// matrix M: clip_i = ((M[i,0] x + M[i,1] y) + M[i,2] z) + M[i,3], the
+// expression dr_transform_homogeneous evaluates (so results are identical to
+// rasterizing its output).
+__device__ __forceinline__ float4 transform_point(const float* __restrict__ M,
+                                                  float x, float y, float z) {
+  float4 r;
+  float* rp = &r.x;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float* m = M + 4 * i;
+    rp[i] = ((m[0] * x + m[1] * y) + m[2] * z) + m[3] * 1.0f;
+  }
+  return r;
+}
+
+// Loads triangle `t` of `job` in clip space; false if an index is out of
+// range.
 __device__ __forceinline__ bool load_triangle(const dr_raster_job& job,
                                               int t, float4 v[3]) {
   const int32_t* tri = job.triangles + 3 * int64_t(t);
-  const float4* verts = reinterpret_cast<const float4*>(job.vertices);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int32_t i = tri[k];
     if (i < 0 || i >= job.num_vertices) return false;
-    v[k] = verts[i];
+    if (job.camera) {
+      const float* p = job.vertices + 3 * int64_t(i);
+      v[k] = transform_point(job.camera, p[0], p[1], p[2]);
+    } else {
+      v[k] = reinterpret_cast<const float4*>(job.vertices)[i];
+    }
   }
   return true;
 }
@@ -956,8 +977,9 @@ int dr_raster_jobs_prepare(dr_raster_job* jobs, int32_t n,
     DR_CHECK_ARG(j.num_triangles == 0 || (j.vertices && j.triangles),
                  "null mesh pointer");
     DR_CHECK_ARG(j.triangle_ids && j.z_buffer, "null output pointer");
-    DR_CHECK_ARG((reinterpret_cast<uintptr_t>(j.vertices) & 15) == 0,
-                 "vertices must be 16-byte aligned");
+    DR_CHECK_ARG(j.camera != nullptr ||
+                     (reinterpret_cast<uintptr_t>(j.vertices) & 15) == 0,
+                 "clip-space vertices must be 16-byte aligned");
     j.triangle_begin = tri;
     j.tile_begin = tiles;
     tri += j.num_triangles;

@@ -111,12 +111,29 @@ __device__ __forceinline__ float center_offset(float c, float w, int size) {
 }
 
 // Interpolated clip x, y, w and the splat's pixel-centre offsets.
+// Clip-space vertex v: the stored clip position, or (without one) the world
+// position transformed as dr_transform_homogeneous does, rows x, y, w only
+// (the splat takes z from the rasterizer's z buffer).
+__device__ __forceinline__ float4 clip_vertex(const dr_splat_job& J, int v) {
+  if (J.clip_vertices)
+    return reinterpret_cast<const float4*>(J.clip_vertices)[v];
+  const float* p = J.positions + 3 * int64_t(v);
+  const float x = p[0], y = p[1], z = p[2];
+  const float* M = J.camera;
+  float4 r;
+  r.x = ((M[0] * x + M[1] * y) + M[2] * z) + M[3] * 1.0f;
+  r.y = ((M[4] * x + M[5] * y) + M[6] * z) + M[7] * 1.0f;
+  r.z = 0.0f;
+  r.w = ((M[12] * x + M[13] * y) + M[14] * z) + M[15] * 1.0f;
+  return r;
+}
+
 __device__ __forceinline__ void splat_position(const dr_splat_job& J,
                                                const Corner& k, float& ox,
                                                float& oy, float& cx,
                                                float& cy, float& cw) {
-  const float4* cv = reinterpret_cast<const float4*>(J.clip_vertices);
-  const float4 p0 = cv[k.v[0]], p1 = cv[k.v[1]], p2 = cv[k.v[2]];
+  const float4 p0 = clip_vertex(J, k.v[0]), p1 = clip_vertex(J, k.v[1]),
+               p2 = clip_vertex(J, k.v[2]);
   cx = interp3(k, p0.x, p1.x, p2.x);
   cy = interp3(k, p0.y, p1.y, p2.y);
   cw = interp3(k, p0.w, p1.w, p2.w);
@@ -801,8 +818,9 @@ int dr_splat_jobs_prepare(dr_splat_job* jobs, int32_t n, int32_t channels,
     DR_CHECK_ARG(j.triangle_ids && j.z_buffer && j.barycentrics,
                  "null rasterizer output");
     DR_CHECK_ARG(j.num_triangles == 0 ||
-                     (j.clip_vertices && j.triangles && j.colors),
-                 "null mesh pointer");
+                     ((j.clip_vertices || (j.positions && j.camera)) &&
+                      j.triangles && j.colors),
+                 "null mesh pointer (clip_vertices, or positions + camera)");
     DR_CHECK_ARG((reinterpret_cast<uintptr_t>(j.clip_vertices) & 15) == 0,
                  "clip_vertices must be 16-byte aligned");
     j.tile_begin = tiles;

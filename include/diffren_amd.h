@@ -56,11 +56,18 @@ const char* dr_version(void);
  * initialises them (rasterize_triangles_cpu.cc:55-60).  `barycentrics` may be
  * NULL.  triangle_begin / tile_begin are filled by dr_raster_jobs_prepare. */
 typedef struct dr_raster_job {
-  const float* vertices;   /* [num_vertices, 4] */
+  const float* vertices;   /* [num_vertices, 4] clip space, 16-byte aligned;
+                              or world [num_vertices, 3] when camera is set */
   const int32_t* triangles; /* [num_triangles, 3] */
   int32_t* triangle_ids;   /* [L, height, width] */
   float* z_buffer;         /* [L, height, width] */
   float* barycentrics;     /* [L, height, width, 3] or NULL */
+  const float* camera;     /* [4, 4] row-major or NULL: with a camera each
+                              vertex load transforms the world position
+                              exactly as dr_transform_homogeneous does, so
+                              no clip-space copy is written (the camera
+                              matmul of rasterize.py:90-96 fused into the
+                              rasterizer) */
   int32_t num_vertices;
   int32_t num_triangles;
   int32_t width;
@@ -277,7 +284,11 @@ int dr_splat_bwd(const float* xyz, const float* color, const float* mask,
  * interpolated colours).  All pointers are device pointers; C = channels,
  * the last channel is alpha. */
 typedef struct dr_splat_job {
-  const float* clip_vertices;   /* [V, 4] clip space, 16-byte aligned */
+  const float* clip_vertices;   /* [V, 4] clip space, 16-byte aligned; or
+                                   NULL to transform `positions` by `camera`
+                                   on load (as dr_transform_homogeneous) */
+  const float* positions;       /* [V, 3] world positions when
+                                   clip_vertices is NULL */
   const int32_t* triangles;     /* [T, 3] */
   const float* colors;          /* [V, C] */
   const float* camera;          /* [4, 4] row-major, or NULL (see d_vertices) */

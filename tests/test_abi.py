@@ -59,3 +59,34 @@ def test_header_enums_match_python():
   text = open(os.path.join(ROOT, "include", "diffren_amd.h")).read()
   assert "DR_CULL_NONE = 0, DR_CULL_BACK = 1, DR_CULL_FRONT = 2" in text
   assert [int(m) for m in constants.FaceCullingMode] == [0, 1, 2]
+
+
+def test_struct_layouts_match_header(tmp_path):
+  """Every ctypes mirror has the header struct's size and field offsets
+  (a C program built from include/diffren_amd.h with gcc prints them)."""
+  import subprocess
+  mirrors = {"dr_raster_job": _native.RasterJob,
+             "dr_raster_config": _native.RasterConfig,
+             "dr_splat_job": _native.SplatJob,
+             "dr_phong_params": _native.PhongParams}
+  lines = ['#include <stddef.h>', '#include <stdio.h>',
+           '#include "diffren_amd.h"', 'int main(void) {']
+  for name, cls in mirrors.items():
+    lines.append(f'  printf("{name} size %zu\\n", sizeof({name}));')
+    for field, _ in cls._fields_:
+      lines.append(f'  printf("{name} {field} %zu\\n", '
+                   f'offsetof({name}, {field}));')
+  lines += ['  return 0;', '}']
+  src = tmp_path / "layout.c"
+  src.write_text("\n".join(lines) + "\n")
+  exe = tmp_path / "layout"
+  subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"),
+                  str(src), "-o", str(exe)], check=True)
+  got = subprocess.run([str(exe)], check=True, capture_output=True,
+                       text=True).stdout.split("\n")
+  want = []
+  for name, cls in mirrors.items():
+    want.append(f"{name} size {ctypes.sizeof(cls)}")
+    for field, _ in cls._fields_:
+      want.append(f"{name} {field} {getattr(cls, field).offset}")
+  assert got[:-1] == want

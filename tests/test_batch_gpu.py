@@ -122,6 +122,44 @@ def test_graph_replays_match_eager(streams, device):
     torch.ones(1000, device=device).sum().item()
 
 
+@pytest.mark.parametrize("streams", [1, 4])
+def test_fused_transform_matches_clip_buffers(streams, device):
+  """Vertex transform on load (no clip-space buffers) vs the transform
+  launch: rasterization and images bit-identical, gradients to float
+  summation order; graph replays too."""
+  meshes = _meshes(device)
+  jobs = _jobs(meshes)
+  a = batch.BatchedSplatRenderer(meshes, jobs, 3, streams=streams)
+  b = batch.BatchedSplatRenderer(meshes, jobs, 3, streams=streams,
+                                 fused_transform=True)
+  assert not b.clip_blocks
+  for ta, tb in zip(a.targets, b.targets):
+    ta.uniform_()
+    tb.copy_(ta)
+  a.step()
+  b.step()
+  torch.cuda.synchronize()
+  for x, y in zip(a.ids + a.images, b.ids + b.images):
+    assert torch.equal(x, y)
+  for ia, za, ba, zb, bb in zip(a.ids, a.z, a.barys, b.z, b.barys):
+    hit = ia >= 0
+    assert helpers.same_bits(za[hit].cpu().numpy(), zb[hit].cpu().numpy())
+    assert helpers.same_bits(ba[hit].cpu().numpy(), bb[hit].cpu().numpy())
+  want = [t.clone() for t in a.d_vertices + a.d_colors]
+  for x, y in zip(want, b.d_vertices + b.d_colors):
+    assert helpers.rel_err(y.cpu(), x.cpu()) <= 1e-5
+  assert abs(a.loss.item() - b.loss.item()) <= 1e-6 * abs(a.loss.item())
+  b.capture()
+  for k in range(3):
+    b.replay()
+    torch.cuda.synchronize()
+    for x, y in zip(want, b.d_vertices + b.d_colors):
+      assert helpers.rel_err(y.cpu(), x.cpu()) <= 1e-5, k
+    for x, y in zip(a.images, b.images):
+      assert torch.equal(x, y), k
+    torch.ones(1000, device=device).sum().item()
+
+
 def test_uneven_batch_autotuned_matches_per_view(device):
   """The batching configuration at reduced scale: meshes of uneven triangle
   counts and resolutions in one batch, autotuned tiles, vs per-view renders
