@@ -58,6 +58,41 @@ def adversarial_scene(seed, num_vertices=40, num_triangles=80):
   return v, t
 
 
+def reference_uv_sphere(n_rows, n_cols, radius):
+  """UV sphere with the vertex and triangle order of
+  rasterize_test_utils.cc:34-118: top pole, n_rows rings of n_cols vertices
+  (phi = pi (r + 1) / (n_rows + 1), theta = 2 pi c / n_cols), bottom pole;
+  top fan, two triangles per quad, bottom fan.  Returns clip-space
+  vertices [V, 4] (w = 1, AddHomogeneousCoord :120-132) and triangles."""
+  r = np.arange(n_rows)[:, None]
+  c = np.arange(n_cols)[None, :]
+  phi = np.broadcast_to((np.pi * (r + 1) / (n_rows + 1)).astype(np.float32),
+                        (n_rows, n_cols))
+  theta = np.broadcast_to((2.0 * np.pi * c / n_cols).astype(np.float32),
+                          (n_rows, n_cols))
+  rad = np.float32(radius)
+  ring = np.stack([rad * np.sin(phi) * np.sin(theta), rad * np.cos(phi),
+                   rad * np.sin(phi) * np.cos(theta)], -1).reshape(-1, 3)
+  xyz = np.concatenate([[[0.0, radius, 0.0]], ring,
+                        [[0.0, -radius, 0.0]]]).astype(np.float32)
+  n_verts = xyz.shape[0]
+  nxt = (np.arange(n_cols) + 1) % n_cols
+  cols = np.arange(n_cols)
+  tris = [np.stack([np.zeros(n_cols, int), 1 + cols, 1 + nxt], 1)]
+  for ri in range(n_rows - 1):
+    tl = 1 + ri * n_cols + cols
+    tr = 1 + ri * n_cols + nxt
+    bl = 1 + (ri + 1) * n_cols + cols
+    br = 1 + (ri + 1) * n_cols + nxt
+    quad = np.stack([tl, bl, br, tr, tl, br], 1).reshape(-1, 3)
+    tris.append(quad)
+  last = n_verts - 1 - n_cols
+  tris.append(np.stack([last + cols, np.full(n_cols, n_verts - 1),
+                        last + nxt], 1))
+  v = np.concatenate([xyz, np.ones((n_verts, 1), np.float32)], 1)
+  return v, np.concatenate(tris).astype(np.int32)
+
+
 def reference_cases():
   """(name, vertices[V,4], triangles[T,3], W, H, L, cull) for every C++ test
   scene of rasterize_triangles_impl_test.cc:125-294."""

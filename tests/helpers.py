@@ -109,3 +109,72 @@ def cpp_images_are_near(baseline, result):
   n = np.count_nonzero(np.abs(baseline.astype(int) - result.astype(int)) > 10)
   limit = int(0.005 * baseline.shape[0] * baseline.shape[1])
   return n <= limit, f"{n} outlier channels, max {limit}"
+
+
+def load_flat_obj(name):
+  """obj_loader.py:24-100 for the fixture .obj files: v / vt / vn / f
+  lines; each distinct face-vertex name becomes one flattened vertex in
+  order of first use; triangles are 0-based.  Returns (vertices [N, 3 + 2
+  if vt + 3 if vn] float32, triangles [M, 3] int32)."""
+  kinds = ("v", "vt", "vn")
+  lists = {k: [] for k in kinds}
+  flat, index, tris = [], {}, []
+  used = {k: False for k in kinds}
+  with open(os.path.join(FIXTURES, name)) as f:
+    for line in f:
+      tok = line.split()
+      if not tok:
+        continue
+      if tok[0] in kinds:
+        lists[tok[0]].append([float(x) for x in tok[1:]])
+      elif tok[0] == "f":
+        tri = []
+        for name_ in tok[1:4]:
+          if name_ not in index:
+            parts = (name_.split("/") + ["", ""])[:3]
+            vert = []
+            for k, i in zip(kinds, parts):
+              if i:
+                vert += lists[k][int(i) - 1]
+                used[k] = True
+              else:
+                vert += [0.0, 0.0] if k == "vt" else [0.0, 0.0, 0.0]
+            index[name_] = len(flat)
+            flat.append(vert)
+          tri.append(index[name_])
+        tris.append(tri)
+  arr = np.asarray(flat, np.float32)
+  out = [arr[:, :3]]
+  if used["vt"]:
+    out.append(arr[:, 3:5])
+  if used["vn"]:
+    out.append(arr[:, -3:])
+  return np.concatenate(out, 1), np.asarray(tris, np.int32)
+
+
+def reference_look_at(view):
+  """test_utils.py:38-77: the test views' look-at matrices (float64)."""
+  eye = {"view_1": (2.0, 3.0, 6.0), "view_2": (-3.0, 1.0, 6.0)}[view]
+  eye = np.array(eye)
+  fwd = -eye / np.linalg.norm(eye)
+  side = np.cross(fwd, (0.0, 1.0, 0.0))
+  side = side / np.linalg.norm(side)
+  up = np.cross(side, fwd)
+  rot = np.eye(4)
+  rot[0, :3], rot[1, :3], rot[2, :3] = side, up, -fwd
+  tr = np.eye(4)
+  tr[:3, 3] = -eye
+  return rot @ tr
+
+
+def reference_perspective():
+  """test_utils.py:80-96 with index 0: fov 40, 640/480, near 0.01, far 10."""
+  f = 1.0 / np.tan(40.0 * (np.pi / 360.0))
+  a = -(10.0 + 0.01) / (10.0 - 0.01)
+  b = -2.0 * (10.0 * 0.01 / (10.0 - 0.01))
+  return np.array([[f / (640 / 480), 0, 0, 0], [0, f, 0, 0], [0, 0, a, b],
+                   [0, 0, -1, 0]])
+
+
+# render_test.py:113-118: the disparity range Sphere_Disparity.png spans.
+DISPARITY_MIN, DISPARITY_MAX = 0.12523349, 0.16625337

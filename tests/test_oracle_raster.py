@@ -100,6 +100,17 @@ def test_oracle_culling(cull, fixture):
   assert (ids[1] == -1).all()
 
 
+def test_oracle_uv_sphere_ids_match_reference_png():
+  # rasterize_triangles_impl_cuda_test.cc:171-187: 230x230 UV sphere of
+  # radius 0.9 at 1600x1600, one layer, no culling, ids encoded by colour.
+  v, t = golden_scenes.reference_uv_sphere(230, 230, 0.9)
+  assert t.shape == (2 * 230 + 2 * 229 * 230, 3)
+  ids, _, _ = raster_oracle.rasterize(v, t, 1600, 1600, 1, 0)
+  ok, msg = helpers.cpp_images_are_near(helpers.load_fixture("UV_Sphere_Ids.png"),
+                                        helpers.encode_ids(ids, 0))
+  assert ok, msg
+
+
 def test_oracle_invariants():
   # Edge pixel (rasterize_triangles_impl_test.cc:170-198), both windings.
   for tri in ([[0, 1, 2]], [[2, 1, 0]]):

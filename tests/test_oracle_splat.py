@@ -109,6 +109,27 @@ def test_unlit_cube_matches_reference_pngs(layers):
       assert ok, (view, l, msg)
 
 
+def test_sphere_disparity_matches_reference_png():
+  # render_test.py:100-141: sphere.obj, OVER compositing, BACK culling, the
+  # shader maps world position to normalised disparity -1 / z_camera.
+  v, t = helpers.load_flat_obj("sphere.obj")
+  look = helpers.reference_look_at("view_1")
+  proj = (helpers.reference_perspective() @ look).astype(np.float32)
+  clip = so.transform(proj, v, torch.float32).numpy()
+  ids, _, b = raster_oracle.rasterize(clip, t, 640, 480, 1, 1)
+  pos = so.interpolate(torch.as_tensor(v), ids, b, t, (0.0,))
+  cam = so.transform(look.astype(np.float32), pos.reshape(-1, 3),
+                     torch.float32)
+  disp = -1.0 / cam[:, 2]
+  disp = (disp - helpers.DISPARITY_MIN) / (helpers.DISPARITY_MAX -
+                                           helpers.DISPARITY_MIN)
+  img = disp.reshape(ids.shape) * torch.as_tensor(ids >= 0,
+                                                  dtype=torch.float32)
+  ok, msg = helpers.check_image(img[0, :, :, None].numpy(),
+                                "Sphere_Disparity.png")
+  assert ok, msg
+
+
 def test_gradient_matches_finite_differences():
   """d loss / d vertices and colours of the float64 oracle vs central
   differences (eps 1e-3) with the rasterization frozen."""

@@ -101,6 +101,18 @@ def test_million_triangle_sphere_bit_exact(device):
   assert (want[0][0] >= 0).mean() > 0.2
 
 
+def test_uv_sphere_ids_match_reference_png(device):
+  # rasterize_triangles_impl_cuda_test.cc:171-187 (230x230 UV sphere at
+  # 1600x1600, no culling): bit-exact to the oracle and within the C++
+  # tests' tolerance of UV_Sphere_Ids.png.
+  v, t = golden_scenes.reference_uv_sphere(230, 230, 0.9)
+  got = gpu_raster(v, t, 1600, 1600, 1, 0, device)
+  assert_same(got, raster_oracle.rasterize(v, t, 1600, 1600, 1, 0))
+  ok, msg = helpers.cpp_images_are_near(
+      helpers.load_fixture("UV_Sphere_Ids.png"), helpers.encode_ids(got[0], 0))
+  assert ok, msg
+
+
 def test_large_triangles_cross_many_tiles(device):
   # Few triangles covering the whole frame: the wave-per-triangle path.
   v = np.array([[-1, -1, 0.2, 1], [1, -1, 0.2, 1], [1, 1, -0.3, 1],
