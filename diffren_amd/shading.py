@@ -7,6 +7,10 @@ phong(...)            -> [...,3] many-light form of shaders.point_light
 texture_map(...)      -> [...,H,W,C] shaders.texture_map
                          (diffren/jax/utils/shaders.py:47-72) on the
                          bilinear resampling kernel, edge clamped
+load_texture(...)     -> shaders.load_texture (:25-34), lower-left origin
+diffuse_light(...)    -> shaders.diffuse_light (:75-97)
+point_light(...)      -> shaders.point_light (:100-173), one light, as the
+                         reference writes it (elementwise torch ops)
 MLPShader             12 -> 64x4 -> 3 MLP, bfloat16 MFMA, float32 accumulation,
                       trained by the library's own forward/backward/Adam
                       kernels; `shade_with_fallback` keeps the Phong result
@@ -46,23 +50,71 @@ def gbuffer(framebuffer, positions, normals, albedo, eye):
   return out
 
 
-def texture_map(uv_image, texture_image):
+def load_texture(texture_filename, lower_left_origin=True):
+  """shaders.py:25-34: an image file as float32 [Ht, Wt, C] in [0, 1], rows
+  flipped so that v = 0 is the bottom row (OpenGL convention)."""
+  from PIL import Image
+  with Image.open(texture_filename) as im:
+    tex = np.asarray(im).astype(np.float32) / 255.0
+  if lower_left_origin:
+    tex = np.ascontiguousarray(np.flipud(tex))
+  return tex
+
+
+def texture_map(uv_image, texture_image_or_filename):
   """Bilinear texture lookup with CLAMP_TO_EDGE (shaders.py:47-72).
 
-  uv_image: [..., H, W, 2] UV coordinates in [0, 1]; texture_image: a
-  [Ht, Wt, C] float tensor (the reference also accepts a file name; decode
-  it with PIL and pass the array).  Returns [..., H, W, C], differentiable
-  with respect to both inputs.
+  uv_image: [..., H, W, 2] UV coordinates in [0, 1]; the texture is a
+  [Ht, Wt, C] float tensor or array, or a file name for load_texture.
+  Returns [..., H, W, C], differentiable with respect to both inputs.
   """
+  texture_image = texture_image_or_filename
   if isinstance(texture_image, str):
-    raise ValueError("texture_map takes a decoded [Ht, Wt, C] texture "
-                     "tensor")
+    texture_image = load_texture(texture_image)
   tex = torch.as_tensor(texture_image, dtype=torch.float32,
                         device=uv_image.device)
   scale = torch.tensor([tex.shape[1], tex.shape[0]], dtype=torch.float32,
                        device=uv_image.device)
   query = uv_image * scale
   return image.bilinear_resample(tex, query, pad_mode="edge")
+
+
+def l2_normalize(x, dim=-1, eps=1e-12):
+  """transforms.py:26-37: x * rsqrt(sum(x^2) + eps) along dim."""
+  return x * torch.rsqrt((x * x).sum(dim=dim, keepdim=True) + eps)
+
+
+def diffuse_light(diffuse_image, normals_image,
+                  light_direction=(0.5, 0.5, 1.0), ka=0.5, kd=0.5):
+  """shaders.py:75-97: ambient plus clamped n.l directional diffuse."""
+  d = l2_normalize(torch.as_tensor(light_direction, dtype=torch.float32,
+                                   device=normals_image.device))
+  n_dot_l = torch.clamp((normals_image * d).sum(-1, keepdim=True), 0.0, 1.0)
+  return diffuse_image * ka + diffuse_image * kd * n_dot_l
+
+
+def point_light(diffuse_image, normals_image, pixel_positions,
+                point_locations, eye_position=(0.0, 0.0, 0.0), ka=0.3,
+                kd=1.4, ks=0.0, ns=0.0, attenuation_constant=1.0,
+                attenuation_linear=0.7, attenuation_quadratic=1.8):
+  """shaders.py:100-173: one point light with ambient, diffuse and
+  Blinn-Phong specular terms and quadratic attenuation (the many-light
+  kernel is `phong`)."""
+  dev = pixel_positions.device
+  light = torch.as_tensor(point_locations, dtype=torch.float32, device=dev)
+  eye = torch.as_tensor(eye_position, dtype=torch.float32, device=dev)
+  to_light = light - pixel_positions
+  light_dir = l2_normalize(to_light)
+  view_dir = l2_normalize(eye - pixel_positions)
+  half = l2_normalize(light_dir + view_dir)
+  nol = torch.clamp((normals_image * light_dir).sum(-1, keepdim=True), 0.0,
+                    1.0)
+  noh = torch.clamp((normals_image * half).sum(-1, keepdim=True), 0.0, 1.0)
+  dist = torch.linalg.vector_norm(to_light, dim=-1, keepdim=True)
+  decay = 1.0 / (attenuation_constant + attenuation_linear * dist +
+                 attenuation_quadratic * dist * dist)
+  luma = ka + kd * nol + ks * noh ** ns
+  return diffuse_image * luma * decay
 
 
 def phong(gbuf, lights, **params):

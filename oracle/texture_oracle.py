@@ -11,9 +11,13 @@ In float32 it mirrors the reference arithmetic (each product and sum as the
 source writes it); in float64 its autograd is the gradient oracle.  The clip
 is `minimum(maximum(t, 0), 1)`, as jnp.clip is written, so its derivative at
 a bound is 1/2 in torch as in JAX.
-Pins (tests/test_oracle_texture.py): the reference PNG fixture
-Textured_Square.png (shaders_test.py:33-71, through the oracle render
-path), closed forms (a texel centre returns the texel, a midpoint the mean
+Also restates the shaders the reference's textured-object tests apply
+after the lookup (shaders.py:75-173 diffuse_light / point_light,
+mesh.py:22-52 vertex normals, transforms.py:26-37 l2_normalize) and the
+texture decode (shaders.py:25-34).
+Pins (tests/test_oracle_texture.py): the reference PNG fixtures
+Textured_Square.png (shaders_test.py:33-71) and Spot_Textured*.png
+(shaders_test.py:75-172), each through the oracle render path, closed forms (a texel centre returns the texel, a midpoint the mean
 of the two texels, edge padding clamps, constant padding fades to 0 over
 the outer half texel, a constant image resamples to itself), and float64
 central finite differences.
@@ -69,3 +73,50 @@ def texture_map(uv_image, texture):
   uv = torch.as_tensor(uv_image).to(tex.dtype)
   scale = torch.tensor([tex.shape[1], tex.shape[0]], dtype=tex.dtype)
   return bilinear_resample(tex, uv * scale, "edge")
+
+
+def load_texture(path):
+  """shaders.py:25-34: uint8 image / 255, rows flipped (lower-left origin)."""
+  from PIL import Image
+  with Image.open(path) as im:
+    return np.ascontiguousarray(np.flipud(np.asarray(im).astype(np.float32) /
+                                          255.0))
+
+
+def l2_normalize(x, eps=1e-12):
+  return x * torch.rsqrt((x * x).sum(-1, keepdim=True) + eps)
+
+
+def vertex_normals(vertices, triangles):
+  """mesh.py:22-52: area-weighted, normalised."""
+  v = torch.as_tensor(vertices)
+  t = torch.as_tensor(np.asarray(triangles)).long()
+  c = v[t]
+  n = torch.linalg.cross(c[:, 1] - c[:, 0], c[:, 2] - c[:, 0])
+  acc = torch.zeros_like(v)
+  for i in range(3):
+    acc = acc + torch.zeros_like(v).index_add_(0, t[:, i], n)
+  return l2_normalize(acc)
+
+
+def diffuse_light(diffuse, normals, light_direction=(0.5, 0.5, 1.0), ka=0.5,
+                  kd=0.5):
+  """shaders.py:75-97."""
+  d = l2_normalize(torch.tensor(light_direction, dtype=normals.dtype))
+  ndl = torch.clamp((normals * d).sum(-1, keepdim=True), 0.0, 1.0)
+  return diffuse * ka + diffuse * kd * ndl
+
+
+def point_light(diffuse, normals, positions, light, eye=(0.0, 0.0, 0.0),
+                ka=0.3, kd=1.4, ks=0.0, ns=0.0, ac=1.0, al=0.7, aq=1.8):
+  """shaders.py:100-173 (one light)."""
+  light = torch.tensor(light, dtype=positions.dtype)
+  eye = torch.tensor(eye, dtype=positions.dtype)
+  ld = l2_normalize(light - positions)
+  vd = l2_normalize(eye - positions)
+  h = l2_normalize(ld + vd)
+  nol = torch.clamp((normals * ld).sum(-1, keepdim=True), 0.0, 1.0)
+  noh = torch.clamp((normals * h).sum(-1, keepdim=True), 0.0, 1.0)
+  dist = torch.linalg.vector_norm(light - positions, dim=-1, keepdim=True)
+  decay = 1.0 / (ac + al * dist + aq * dist * dist)
+  return diffuse * (ka + kd * nol + ks * noh ** ns) * decay

@@ -155,9 +155,14 @@ def load_flat_obj(name):
 def reference_look_at(view):
   """test_utils.py:38-77: the test views' look-at matrices (float64)."""
   eye = {"view_1": (2.0, 3.0, 6.0), "view_2": (-3.0, 1.0, 6.0)}[view]
+  return reference_look_at_eye(eye, (0.0, 1.0, 0.0))
+
+
+def reference_look_at_eye(eye, world_up):
+  """test_utils.py:55-77 for an eye position looking at the origin."""
   eye = np.array(eye)
   fwd = -eye / np.linalg.norm(eye)
-  side = np.cross(fwd, (0.0, 1.0, 0.0))
+  side = np.cross(fwd, world_up)
   side = side / np.linalg.norm(side)
   up = np.cross(side, fwd)
   rot = np.eye(4)

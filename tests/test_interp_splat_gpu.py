@@ -143,3 +143,130 @@ def test_render_splat_forward_and_gradients(layers, device):
   l64.backward()
   assert helpers.rel_err(col.grad.cpu(), c64.grad) <= GRAD_TOL
   assert helpers.rel_err(v.grad.cpu(), v64.grad) <= GRAD_TOL
+
+
+@pytest.mark.parametrize("num_components", [4, 2])
+def test_optimize_single_triangle(num_components, device):
+  """splat_test.py:84-168: gradient descent on a triangle's clip-space
+  vertices toward a translated copy.  Plain rasterization gives a zero
+  vertex gradient; rasterize-then-splat gives a useful one, and 15 steps of
+  unit-rate descent halve the splat loss."""
+  from diffren_amd import constants
+  w = h = 32
+  init = torch.tensor([[0, 0, 0, 1], [0.5, 0, 0, 1], [0.5, 0.5, 0, 1]],
+                      dtype=torch.float32, device=device)
+  target_v = torch.tensor([[-0.25, 0, 0, 1], [0.25, 0, 0, 1],
+                           [0.25, 0.5, 0, 1]], dtype=torch.float32,
+                          device=device)
+  tris = torch.tensor([[0, 1, 2]], dtype=torch.int32, device=device)
+  colors = torch.tensor([[0, 0., 0.8, 1.0], [0, 0.8, 0, 1.0],
+                         [0, 0., 0.8, 1.0]], dtype=torch.float32,
+                        device=device)[:, -num_components:]
+
+  def draw(v, do_splat):
+    mode = (constants.CompositingMode.SPLAT_OVER if do_splat else
+            constants.CompositingMode.OVER)
+    return render.render_triangles(v, {"colors": colors}, tris, None, w, h,
+                                   lambda a: a["colors"],
+                                   compositing_mode=mode)
+
+  target = draw(target_v, False).detach()
+
+  def grads(v):
+    out = []
+    for do_splat in (False, True):
+      vv = v.clone().requires_grad_(True)
+      loss = ((draw(vv, do_splat) - target) ** 2).mean()
+      # Rasterization is not differentiable: the plain image does not
+      # depend on the vertices through any differentiable path.
+      if loss.requires_grad:
+        loss.backward()
+      g = vv.grad if vv.grad is not None else torch.zeros_like(vv)
+      out.append((loss.item(), g))
+    return out
+
+  v = init.clone()
+  initial = None
+  for i in range(15):
+    (_, g_raster), (splat_loss, g_splat) = grads(v)
+    if i == 0:
+      assert torch.linalg.norm(g_raster).item() == 0.0
+      assert torch.linalg.norm(g_splat).item() > 0.01
+      initial = splat_loss
+    v = v - g_splat
+  final = ((draw(v, True) - target) ** 2).mean().item()
+  print(f"splat loss {initial:.5f} -> {final:.5f}")
+  assert final < 0.5 * initial
+
+
+def test_spike_derivative(device):
+  """splat_test.py:170-224 on the GPU path.  The hand-written backward is
+  checked against the oracle's float64 forward-mode derivative J 1 (d image
+  / d spike offset) through the adjoint identity <u, J 1> = (J^T u) . 1 for
+  seeded u; the derivative image from central differences of the GPU
+  forward pass (rasterization frozen, as stop_gradient freezes it) matches
+  the oracle's and Spike_Derivative.png at pixel error threshold 0."""
+  from diffren_amd import constants
+  from test_oracle_splat import spike_scene, spike_visualisation
+  v, t, proj, ids, b = spike_scene()
+  base = torch.as_tensor(v, dtype=torch.float64)
+  cols64 = torch.ones((v.shape[0], 4), dtype=torch.float64)
+  p64 = torch.as_tensor(proj, dtype=torch.float64)
+
+  def f64(off):
+    verts = torch.cat([base[:32], base[32:33] * off, base[33:]], 0)
+    return so.render_splat(verts, cols64, p64, ids, b, t)
+
+  img64, jac64 = torch.autograd.functional.jvp(
+      f64, (torch.tensor([2.0], dtype=torch.float64),),
+      (torch.ones(1, dtype=torch.float64),))
+
+  vt = torch.as_tensor(v, device=device)
+  tt = torch.as_tensor(t, device=device)
+  pt = torch.as_tensor(proj, device=device)
+  cols = torch.ones((v.shape[0], 4), dtype=torch.float32, device=device)
+
+  def spiked(off):
+    return torch.cat([vt[:32], vt[32:33] * off, vt[33:]], 0)
+
+  off = torch.tensor([2.0], device=device, requires_grad=True)
+  img = render.render_triangles(spiked(off), {"c": cols}, tt, pt, 640, 480,
+                                lambda a: a["c"])
+  # Forward: against the float32 oracle (same arithmetic type).
+  sp32 = torch.as_tensor(v).clone()
+  sp32[32] *= 2.0
+  img32 = so.render_splat(sp32, cols64.float(), torch.as_tensor(proj), ids, b,
+                          t)
+  assert helpers.rel_err(img.detach().cpu(), img32) <= FWD_TOL
+  rng = np.random.default_rng(11)
+  for k in range(3):
+    u = rng.normal(size=tuple(img.shape))
+    off.grad = None
+    (img * torch.as_tensor(u, dtype=torch.float32, device=device)).sum(
+    ).backward(retain_graph=True)
+    want = float((torch.as_tensor(u) * jac64).sum())
+    scale = float((torch.as_tensor(u).abs() * jac64.abs()).sum())
+    assert abs(off.grad.item() - want) <= GRAD_TOL * scale, (k, want)
+
+  fb = rasterize.rasterize_triangles(spiked(torch.tensor([2.0],
+                                                         device=device)),
+                                     tt, pt, 640, 480,
+                                     face_culling_mode=constants.
+                                     FaceCullingMode.BACK)
+  shaded = interpolate.interpolate_vertex_attributes({"c": cols}, fb,
+                                                     (0.0,))["c"]
+  # Taps move between pixels as the offset changes (the splat is piecewise
+  # smooth): a step of 3e-4 stays within one piece for nearly every pixel.
+  h = 3e-4
+
+  def f32(o):
+    with torch.no_grad():
+      return splat.splat_shaded_pixels(
+          spiked(torch.tensor([o], device=device)), pt, fb, shaded).double()
+
+  fd = ((f32(2.0 + h) - f32(2.0 - h)) / (2 * h)).cpu()
+  assert helpers.rel_err(fd, jac64) <= 1e-2
+  ok, msg = helpers.check_image(
+      spike_visualisation(img.detach().cpu().double(), fd).numpy(),
+      "Spike_Derivative.png", exact=True)
+  assert ok, msg

@@ -109,6 +109,88 @@ def test_unlit_cube_matches_reference_pngs(layers):
       assert ok, (view, l, msg)
 
 
+@pytest.mark.parametrize("num_components", [4, 2])
+def test_oracle_optimizes_single_triangle(num_components):
+  """splat_test.py:84-168 on the oracle: the plain-raster loss has no vertex
+  gradient, the splat loss has one, and 15 unit-rate descent steps on the
+  float64 gradient halve the splat loss."""
+  init = np.array([[0, 0, 0, 1], [0.5, 0, 0, 1], [0.5, 0.5, 0, 1]],
+                  np.float32)
+  target_v = np.array([[-0.25, 0, 0, 1], [0.25, 0, 0, 1], [0.25, 0.5, 0, 1]],
+                      np.float32)
+  tris = np.array([[0, 1, 2]], np.int32)
+  colors = torch.tensor([[0, 0., 0.8, 1.0], [0, 0.8, 0, 1.0],
+                         [0, 0., 0.8, 1.0]],
+                        dtype=torch.float64)[:, -num_components:]
+
+  def draw(v, do_splat):
+    ids, _, b = raster_oracle.rasterize(
+        v.detach().numpy().astype(np.float32), tris, 32, 32, 1, 1)
+    shaded = so.interpolate(colors, ids, b, tris, (0.0,))
+    if do_splat:
+      return so.splat_shaded_pixels(v, ids, b, tris, shaded)[0]
+    return so.over(shaded * torch.as_tensor(ids >= 0,
+                                            dtype=torch.float64)[..., None])
+
+  target = draw(torch.as_tensor(target_v, dtype=torch.float64), False)
+  v = torch.as_tensor(init, dtype=torch.float64)
+  initial = None
+  for i in range(15):
+    vv = v.clone().requires_grad_(True)
+    raster_loss = ((draw(vv, False) - target) ** 2).mean()
+    assert not raster_loss.requires_grad
+    loss = ((draw(vv, True) - target) ** 2).mean()
+    loss.backward()
+    if i == 0:
+      assert torch.linalg.norm(vv.grad).item() > 0.01
+      initial = loss.item()
+    v = v - vv.grad
+  final = ((draw(v, True) - target) ** 2).mean().item()
+  assert final < 0.5 * initial, (initial, final)
+
+
+def spike_scene():
+  """splat_test.py:170-196: sphere.obj with vertex 32 scaled by an offset
+  (2.0), view_1 projection, white RGBA colours, 640x480, one layer, BACK
+  culling; rasterization of the spiked mesh, frozen (stop_gradient)."""
+  v, t = helpers.load_flat_obj("sphere.obj")
+  proj = (helpers.reference_perspective().astype(np.float32).astype(
+      np.float64) @ helpers.reference_look_at("view_1").astype(
+          np.float32).astype(np.float64)).astype(np.float32)
+  spiked = v.copy()
+  spiked[32] *= 2.0
+  clip = so.transform(proj, spiked, torch.float32).numpy()
+  ids, _, b = raster_oracle.rasterize(clip, t, 640, 480, 1, 1)
+  return v, t, proj, ids, b
+
+
+def spike_visualisation(image, jacobian):
+  """splat_test.py:201-209: the normalised JVP over a faint image."""
+  nj = (jacobian / jacobian.abs().max()) * 0.5 + 0.5
+  return (nj * 0.95 + image * 0.05)[..., :3]
+
+
+def test_oracle_spike_derivative_matches_reference_png():
+  # splat_test.py:170-224: the forward-mode derivative of the image with
+  # respect to the spike offset, at pixel error threshold 0.
+  v, t, proj, ids, b = spike_scene()
+  base = torch.as_tensor(v, dtype=torch.float64)
+  cols = torch.ones((v.shape[0], 4), dtype=torch.float64)
+
+  def f(off):
+    verts = torch.cat([base[:32], base[32:33] * off, base[33:]], 0)
+    return so.render_splat(verts, cols,
+                           torch.as_tensor(proj, dtype=torch.float64), ids, b,
+                           t)
+
+  img, jac = torch.autograd.functional.jvp(
+      f, (torch.tensor([2.0], dtype=torch.float64),),
+      (torch.ones(1, dtype=torch.float64),))
+  ok, msg = helpers.check_image(spike_visualisation(img, jac).numpy(),
+                                "Spike_Derivative.png", exact=True)
+  assert ok, msg
+
+
 def test_sphere_disparity_matches_reference_png():
   # render_test.py:100-141: sphere.obj, OVER compositing, BACK culling, the
   # shader maps world position to normalised disparity -1 / z_camera.

@@ -4,6 +4,8 @@ on the GPU vs the CPU oracle (oracle/texture_oracle.py).
 Tolerances: forward bit-exact to the float32 oracle (same expressions, same
 order); gradients within 1e-5 relative of the float64 oracle.
 """
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -16,6 +18,8 @@ from oracle import texture_oracle as to
 from oracle import raster_oracle
 from oracle import splat_oracle as so
 from test_oracle_texture import SQUARE, SQUARE_TEXTURE, SQUARE_TRIS, SQUARE_UV
+from test_oracle_texture import boundary_inputs
+from test_oracle_texture import SPOT_CASES, spot_oracle, spot_scene, spot_shade
 
 pytestmark = pytest.mark.gpu
 
@@ -80,6 +84,45 @@ def test_chw_orders(device):
     image.bilinear_resample(img, loc[..., :1])
 
 
+@pytest.mark.parametrize("order", ["HWC", "CHW"])
+def test_recovers_input(order, device):
+  # image_test.py:25-44.
+  n = 64
+  img = np.arange(3 * n * n, dtype=np.float32).reshape(n, n, 3)
+  x, y = np.meshgrid(np.arange(n), np.arange(n))
+  loc = (np.stack((x, y), -1) + 0.5).astype(np.float32)
+  if order == "CHW":
+    img_in, loc_in = img.transpose(2, 0, 1), loc.transpose(2, 0, 1)
+  else:
+    img_in, loc_in = img, loc
+  out = image.bilinear_resample(
+      torch.as_tensor(np.ascontiguousarray(img_in), device=device),
+      torch.as_tensor(np.ascontiguousarray(loc_in), device=device),
+      image_order=order, locations_order=order).cpu().numpy()
+  assert np.array_equal(out, img_in)
+
+
+@pytest.mark.parametrize("order", ["HWC", "CHW"])
+def test_boundary_cases(order, device):
+  # image_test.py:46-108, infinities included; bit-exact to the oracle.
+  img, loc, want = boundary_inputs()
+  if order == "CHW":
+    img_in, loc_in = img.transpose(2, 0, 1), loc.transpose(2, 0, 1)
+    want = want.transpose(2, 0, 1)
+  else:
+    img_in, loc_in = img, loc
+  out = image.bilinear_resample(
+      torch.as_tensor(np.ascontiguousarray(img_in), device=device),
+      torch.as_tensor(np.ascontiguousarray(loc_in), device=device),
+      image_order=order, locations_order=order).cpu().numpy()
+  np.testing.assert_array_almost_equal(want, out, decimal=5)
+  ref = to.bilinear_resample(torch.as_tensor(img), torch.as_tensor(loc))
+  ref = ref.numpy()
+  if order == "CHW":
+    ref = ref.transpose(2, 0, 1)
+  assert helpers.same_bits(out, np.ascontiguousarray(ref))
+
+
 def test_textured_square_matches_reference_png(device):
   # shaders_test.py:33-71 through the GPU render path (SPLAT_OVER).
   tex = torch.as_tensor(SQUARE_TEXTURE, device=device)
@@ -116,3 +159,30 @@ def test_texture_gradient_through_render(device):
   ((o64 - target.cpu().double()) ** 2).mean().backward()
   err = helpers.rel_err(t.grad.cpu(), t64.grad)
   assert err <= 1e-4
+
+
+@pytest.mark.parametrize("kind,fixture", SPOT_CASES)
+def test_spot_textured_matches_reference_png(kind, fixture, device):
+  # shaders_test.py:75-172 through the GPU render path (OVER): texture
+  # lookup on the resampling kernel, the library's lights and normals.
+  from diffren_amd import constants
+  from diffren_amd import mesh
+  pos, uv, t, proj = spot_scene()
+  p = torch.as_tensor(pos, device=device)
+  attrs = {"uvs": torch.as_tensor(uv, device=device), "positions": p,
+           "normals": mesh.compute_vertex_normals(
+               p, torch.as_tensor(t, device=device))}
+  path = os.path.join(helpers.FIXTURES, "spot_texture.png")
+
+  def shader(a):
+    textured = shading.texture_map(a["uvs"], path)
+    return spot_shade(kind, textured, a["normals"], a["positions"],
+                      (shading.diffuse_light, shading.point_light))
+
+  img = render.render_triangles(
+      p, attrs, torch.as_tensor(t, device=device),
+      torch.as_tensor(proj, device=device), 640, 480, shader,
+      compositing_mode=constants.CompositingMode.OVER)
+  ok, msg = helpers.check_image(img.cpu().numpy(), fixture)
+  assert ok, msg
+  assert helpers.rel_err(img.cpu(), spot_oracle(kind)) <= 1e-5
