@@ -3,7 +3,8 @@
 Restates diffren/jax/camera.py: `perspective` (:23-44),
 `perspective_from_intrinsics` (:47-92), `ortho` (:95-123) and `look_at`
 (:126-184), gluPerspective / glOrtho / gluLookAt conventions.  Matrices are
-returned as float32 [4, 4] for column vectors (clip = M @ [x, y, z, 1]).
+returned as float64 [4, 4] for column vectors (clip = M @ [x, y, z, 1]);
+cast to float32 before handing them to the render path.
 """
 import math
 

@@ -270,3 +270,24 @@ def test_spike_derivative(device):
       spike_visualisation(img.detach().cpu().double(), fd).numpy(),
       "Spike_Derivative.png", exact=True)
   assert ok, msg
+
+
+@pytest.mark.parametrize("batched", [False, True])
+@pytest.mark.parametrize("channels", [3, 4])
+def test_transform_homogeneous_shapes(batched, channels, device):
+  # transforms_test.py:31-49, plus values against the oracle.
+  rng = np.random.default_rng(channels)
+  shape = (3, 10, channels) if batched else (10, channels)
+  v = rng.normal(size=shape).astype(np.float32)
+  m = rng.normal(size=((3, 4, 4) if batched else (4, 4))).astype(np.float32)
+  got = transforms.transform_homogeneous(torch.as_tensor(m, device=device),
+                                         torch.as_tensor(v, device=device))
+  assert tuple(got.shape) == shape[:-1] + (4,)
+  for i in range(3 if batched else 1):
+    mi, vi, gi = (m[i], v[i], got[i]) if batched else (m, v, got)
+    want = so.transform(mi, vi, torch.float32).numpy()
+    assert helpers.same_bits(gi.cpu().numpy(), want)
+  with pytest.raises(ValueError):
+    transforms.transform_homogeneous(torch.as_tensor(m, device=device),
+                                     torch.as_tensor(v[..., :2],
+                                                     device=device))

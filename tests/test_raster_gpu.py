@@ -205,3 +205,33 @@ def test_extreme_image_shapes(w, h, device):
   v, t = golden_scenes.adversarial_scene(900 + w, 50, 200)
   assert_same(gpu_raster(v, t, w, h, 3, 0, device),
               raster_oracle.rasterize(v, t, w, h, 3, 0))
+
+
+@pytest.mark.parametrize("batched", [False, True])
+@pytest.mark.parametrize("axes", [(None, None), (None, 0), (0, None), (0, 0)])
+def test_primitive_batch_axes_simple_triangle(axes, batched, device):
+  # rasterize_triangles_xla_test.py:35-85: the primitive with and without
+  # leading batch axes on vertices and / or triangles renders
+  # Simple_Triangle.png.
+  v = torch.tensor([[-0.5, -0.5, 0.8, 1.0], [0.0, 0.5, 0.3, 1.0],
+                    [0.5, -0.5, 0.3, 1.0]], device=device)
+  t = torch.tensor([[0, 1, 2]], dtype=torch.int32, device=device)
+  if axes[0] is not None:
+    v = v.unsqueeze(0)
+  if axes[1] is not None:
+    t = t.unsqueeze(0)
+  ids, z, b = rasterize.rasterize_triangles_primitive(v, t, 640, 480, 1, 0,
+                                                      batched=batched)
+  lead = 1 if axes != (None, None) else 0
+  assert tuple(b.shape) == (1,) * lead + (1, 480, 640, 3)
+  b = b.reshape(480, 640, 3).cpu().numpy()
+  rgba = np.concatenate([b, np.ones((480, 640, 1), np.float32)], 2)
+  ok, msg = helpers.check_image(rgba, "Simple_Triangle.png")
+  assert ok, msg
+  want = raster_oracle.rasterize(v.reshape(3, 4).cpu().numpy(),
+                                 t.reshape(1, 3).cpu().numpy(), 640, 480, 1,
+                                 0)
+  assert_same((ids.reshape(1, 480, 640).cpu().numpy(),
+               z.reshape(1, 480, 640).cpu().numpy(),
+               b.reshape(1, 480, 640, 3)), want)
+
