@@ -336,18 +336,20 @@ class BatchedSplatRenderer:
 
   # -- graphs ----------------------------------------------------------------
 
-  def capture(self, warmup=2):
-    """Captures step() in a HIP graph; returns it."""
+  def capture(self, warmup=2, forward_only=False):
+    """Captures step() (or render() when forward_only) in a HIP graph;
+    replay() runs it.  Returns the graph."""
+    fn = self.render if forward_only else self.step
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(s):
       for _ in range(warmup):
-        self.step()
+        fn()
     torch.cuda.current_stream().wait_stream(s)
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
-      self.step()
+      fn()
     torch.cuda.synchronize()
     self.graph = g
     return g

@@ -160,6 +160,38 @@ def test_fused_transform_matches_clip_buffers(streams, device):
     torch.ones(1000, device=device).sum().item()
 
 
+@pytest.mark.parametrize("channels", [1, 12])
+def test_fused_channel_counts(channels, device):
+  """The fused path at other channel counts (12: normals + 8 features +
+  alpha, the forward-only workload's pass-through shader) vs the unfused
+  path, forward and backward; forward-only graph replays too."""
+  meshes = _meshes(device)
+  gen = torch.Generator(device="cpu").manual_seed(channels)
+  for m in meshes:
+    V = m["vertices"].shape[0]
+    m["colors"] = torch.rand((V, channels), generator=gen).to(device)
+  jobs = _jobs(meshes)
+  r = batch.BatchedSplatRenderer(meshes, jobs, 3)
+  for t in r.targets:
+    t.copy_(torch.rand(t.shape, generator=gen))
+  r.step()
+  torch.cuda.synchronize()
+  imgs, loss, dv, dc = _unfused(meshes, jobs, 3, r.targets)
+  for a, b in zip(r.images, imgs):
+    assert helpers.rel_err(a.cpu(), b.detach().cpu()) <= 1e-5
+  assert abs(r.loss.item() - loss.item()) <= 1e-5 * abs(loss.item())
+  for a, b in zip(r.d_vertices + r.d_colors, dv + dc):
+    assert helpers.rel_err(a.cpu(), b.cpu()) <= 1e-4
+  want = [i.clone() for i in r.images]
+  for i in r.images:
+    i.zero_()
+  r.capture(forward_only=True)
+  r.replay()
+  torch.cuda.synchronize()
+  for a, b in zip(r.images, want):
+    assert torch.equal(a, b)
+
+
 def test_uneven_batch_autotuned_matches_per_view(device):
   """The batching configuration at reduced scale: meshes of uneven triangle
   counts and resolutions in one batch, autotuned tiles, vs per-view renders
