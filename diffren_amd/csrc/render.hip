@@ -216,6 +216,11 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
   const int64_t P = int64_t(W) * H;
   const int tid = threadIdx.y * kBX + threadIdx.x;
   constexpr int kThreads = kBX * kBY;
+  // A job without the inputs of the requested gradient (target for mode 1,
+  // d_image for mode 2, grad_scratch for both) gets its image only.
+  const int mode =
+      grad_mode == 1 ? (J.target && J.grad_scratch ? 1 : 0)
+                     : (grad_mode == 2 && J.d_image && J.grad_scratch ? 2 : 0);
 
   // Ids and z of every layer (z buffer read only under a triangle).
   if (tid < kMaxLayers) layer_live[tid] = 0;
@@ -359,18 +364,18 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
       o = n[1][c] + (1.0f - a1) * o;
       o = n[0][c] + (1.0f - a0) * o;
       if (J.image) J.image[p * C + c] = o;
-      if (grad_mode == 1) {
+      if (mode == 1) {
         const float d = o - J.target[p * C + c];
         partial += d * d;
         g[c] = (2.0f * d) * loss_scale;
-      } else if (grad_mode == 2) {
+      } else if (mode == 2) {
         g[c] = J.d_image[p * C + c];
       }
     }
     // A pixel that received no splat weight is never read by the source
     // pass (every foreground source adds a positive weight to each of its
     // 9 targets), so its gradient scratch is left unwritten.
-    if (grad_mode != 0 && (acc[0][C] + acc[1][C]) + acc[2][C] > 0.0f) {
+    if (mode != 0 && (acc[0][C] + acc[1][C]) + acc[2][C] > 0.0f) {
       float dn[3][C];
       float da0 = 0.0f, da1 = 0.0f;
 #pragma unroll
@@ -521,6 +526,9 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   int* nrows = &row_count[wave];
   int* hkey = hash_key[wave];
   int* hrow = hash_row[wave];
+  // Nothing to differentiate into, or no gradient scratch to read: the job
+  // contributes nothing (one job per block, so the whole block leaves).
+  if (J.d_records == nullptr || J.grad_scratch == nullptr) return;
   const bool inside = x < W && y < H;
   const int64_t P = int64_t(W) * H;
   const int64_t p = int64_t(y) * W + x;
@@ -802,13 +810,22 @@ int check(int n, int64_t tiles, int L, int C) {
 }  // namespace
 }  // namespace dr
 
+namespace dr {
+namespace {
+// Channel counts the fused kernels are instantiated for.
+bool render_channels_ok(int c) {
+  return (c >= 1 && c <= 8) || c == 12 || c == 16;
+}
+}  // namespace
+}  // namespace dr
+
 extern "C" {
 
 int dr_splat_jobs_prepare(dr_splat_job* jobs, int32_t n, int32_t channels,
                           int64_t* total_tiles) {
   DR_CHECK_ARG(jobs != nullptr && n > 0, "need at least one job");
-  DR_CHECK_ARG(channels >= 1 && channels <= dr::kMaxChannels,
-               "bad channel count");
+  DR_CHECK_ARG(dr::render_channels_ok(channels),
+               "unsupported channel count (1-8, 12 or 16)");
   int64_t tiles = 0;
   for (int i = 0; i < n; ++i) {
     dr_splat_job& j = jobs[i];
@@ -823,6 +840,8 @@ int dr_splat_jobs_prepare(dr_splat_job* jobs, int32_t n, int32_t channels,
                  "null mesh pointer (clip_vertices, or positions + camera)");
     DR_CHECK_ARG((reinterpret_cast<uintptr_t>(j.clip_vertices) & 15) == 0,
                  "clip_vertices must be 16-byte aligned");
+    DR_CHECK_ARG((reinterpret_cast<uintptr_t>(j.grad_scratch) & 15) == 0,
+                 "grad_scratch must be 16-byte aligned");
     j.tile_begin = tiles;
     tiles += int64_t((j.width + dr::kBX - 1) / dr::kBX) *
              ((j.height + dr::kBY - 1) / dr::kBY);
@@ -832,12 +851,12 @@ int dr_splat_jobs_prepare(dr_splat_job* jobs, int32_t n, int32_t channels,
 }
 
 int32_t dr_render_scratch_floats(int32_t channels) {
-  if (channels < 1 || channels > dr::kMaxChannels) return 0;
+  if (!dr::render_channels_ok(channels)) return 0;
   return 3 * (((channels + 1) + 3) & ~3) + 4;
 }
 
 int32_t dr_render_record_size(int32_t channels) {
-  if (channels < 1 || channels > dr::kMaxChannels) return 0;
+  if (!dr::render_channels_ok(channels)) return 0;
   return (4 + channels) <= 8 ? 8 : ((4 + channels) <= 16 ? 16 : 32);
 }
 

@@ -317,7 +317,8 @@ typedef struct dr_splat_job {
 int dr_splat_jobs_prepare(dr_splat_job* jobs_host, int32_t num_jobs,
                           int32_t channels, int64_t* total_tiles);
 
-/* Floats per pixel of a job's grad_scratch (0 if unsupported). */
+/* Floats per pixel of a job's grad_scratch (0 if the channel count is not
+ * one of 1-8, 12, 16). */
 int32_t dr_render_scratch_floats(int32_t channels);
 
 /* Floats per vertex gradient record for `channels` colour channels (8, 16
@@ -335,7 +336,9 @@ size_t dr_render_splat_workspace_size(int64_t total_tiles);
  * then in a fixed order, so it is deterministic) and its gradient
  * 2 (image - target) loss_scale pushed into grad_scratch.  2: the given
  * d_image pushed into grad_scratch.  channels 1-8, 12 or 16.  The workspace
- * is needed for grad_mode 1 only.
+ * is needed for grad_mode 1 only.  A job whose target (mode 1) or d_image
+ * (mode 2) or grad_scratch is NULL gets its image only (no loss, no
+ * gradient).
  * No reference counterpart as one call: it fuses render.py:123-132 for an
  * identity shader, batched where the reference runs one launch per image. */
 int dr_render_splat_forward(const dr_splat_job* jobs_device, int32_t num_jobs,
@@ -345,7 +348,8 @@ int dr_render_splat_forward(const dr_splat_job* jobs_device, int32_t num_jobs,
                             size_t workspace_bytes, dr_stream_t stream);
 
 /* Source pass: from grad_scratch (written by a forward with grad_mode 1 or
- * 2), accumulate the d_records of all jobs (float atomics).  The
+ * 2), accumulate the d_records of all jobs (float atomics); jobs with a NULL
+ * d_records or grad_scratch are skipped.  The
  * hand-written VJP of splat.py:178-229 + interpolate.py:24-63 +
  * transforms.py:40-78; the reference uses XLA autodiff. */
 int dr_render_splat_backward(const dr_splat_job* jobs_device,

@@ -90,3 +90,31 @@ def test_struct_layouts_match_header(tmp_path):
     for field, _ in cls._fields_:
       want.append(f"{name} {field} {getattr(cls, field).offset}")
   assert got[:-1] == want
+
+
+def test_render_channel_counts_and_job_checks():
+  lib = _native.lib()
+  for c in (1, 4, 8, 12, 16):
+    assert lib.dr_render_record_size(c) > 0
+    assert lib.dr_render_scratch_floats(c) > 0
+  for c in (0, 9, 11, 13, 17):
+    assert lib.dr_render_record_size(c) == 0
+    assert lib.dr_render_scratch_floats(c) == 0
+  job = (_native.SplatJob * 1)()
+  job[0].width = 8
+  job[0].height = 8
+  job[0].triangle_ids = 256
+  job[0].z_buffer = 256
+  job[0].barycentrics = 256
+  tiles = ctypes.c_int64()
+  assert lib.dr_splat_jobs_prepare(job, 1, 4, ctypes.byref(tiles)) == 0
+  assert tiles.value == 2 * 1  # 64x4 tiles: one column, two rows
+  assert lib.dr_splat_jobs_prepare(job, 1, 9, ctypes.byref(tiles)) != 0
+  assert b"channel" in lib.dr_last_error()
+  job[0].grad_scratch = 260
+  assert lib.dr_splat_jobs_prepare(job, 1, 4, ctypes.byref(tiles)) != 0
+  assert b"grad_scratch" in lib.dr_last_error()
+  job[0].grad_scratch = 0
+  job[0].num_triangles = 3
+  assert lib.dr_splat_jobs_prepare(job, 1, 4, ctypes.byref(tiles)) != 0
+  assert b"null mesh" in lib.dr_last_error()

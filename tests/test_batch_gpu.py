@@ -257,3 +257,54 @@ def test_invalid_batches_raise(device):
     batch.BatchedSplatRenderer(meshes, [(5, jobs[0][1], 16, 16)], 1)
   with pytest.raises(ValueError):
     batch.BatchedSplatRenderer(meshes, [(0, jobs[0][1], 0, 16)], 1)
+
+
+def test_jobs_without_gradient_inputs(device):
+  """A job with no target (grad_mode 1) or d_image (grad_mode 2) renders its
+  image only; the source pass skips jobs with no d_records."""
+  import ctypes
+  from diffren_amd import _native
+  meshes = _meshes(device)
+  jobs = _jobs(meshes)[:1]
+  r = batch.BatchedSplatRenderer(meshes, jobs, 3)
+  r.targets[0].uniform_()
+  r.step()
+  torch.cuda.synchronize()
+  want = r.images[0].clone()
+  lib = _native.lib()
+  table = (_native.SplatJob * 1)()
+  mi, _, w, h = jobs[0]
+  m = meshes[mi]
+  j = table[0]
+  j.clip_vertices = r.clip[0].data_ptr()
+  j.triangles = m["triangles"].data_ptr()
+  j.colors = m["colors"].data_ptr()
+  j.camera = r.cameras[0].data_ptr()
+  j.triangle_ids = r.ids[0].data_ptr()
+  j.z_buffer = r.z[0].data_ptr()
+  j.barycentrics = r.barys[0].data_ptr()
+  image = torch.zeros_like(want)
+  j.image = image.data_ptr()
+  j.grad_scratch = r.scratch[0].data_ptr()
+  j.num_vertices = m["vertices"].shape[0]
+  j.num_triangles = m["triangles"].shape[0]
+  j.width, j.height = w, h
+  tiles = ctypes.c_int64()
+  _native.check(lib.dr_splat_jobs_prepare(table, 1, r.C, ctypes.byref(tiles)),
+                "prepare")
+  dev_table = _native.table_to_device(table, device)
+  ws = _native.workspace(lib.dr_render_splat_workspace_size(tiles.value),
+                         device)
+  loss = torch.zeros(1, device=device)
+  for mode in (1, 2):
+    image.zero_()
+    _native.check(lib.dr_render_splat_forward(
+        _native.ptr(dev_table), 1, tiles.value, 3, r.C, mode, 1.0,
+        _native.ptr(loss), _native.ptr(ws), ws.numel(),
+        _native.stream_handle()), "forward")
+    _native.check(lib.dr_render_splat_backward(
+        _native.ptr(dev_table), 1, tiles.value, 3, r.C,
+        _native.stream_handle()), "backward")
+    torch.cuda.synchronize()
+    assert torch.equal(image, want)
+    assert loss.item() == 0.0
