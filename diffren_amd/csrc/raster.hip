@@ -395,21 +395,33 @@ __global__ __launch_bounds__(kBlock) void bin_kernel(
     const uint32_t tile =
         has ? tile0 + uint32_t(k / nx) * uint32_t(ntx) + uint32_t(k % nx)
             : 0xffffffffu;
+    // Group the lanes by tile with ballots alone (no memory traffic): each
+    // lane learns the lowest lane of its group, its rank in the group and
+    // the group size.
     uint64_t todo = __ballot(has);
+    int leader = 0, rank = 0, size = 0;
     while (todo) {
-      const int leader = __ffsll(static_cast<unsigned long long>(todo)) - 1;
-      const uint32_t lt = __builtin_amdgcn_readlane(tile, leader);
+      const int first = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+      const uint32_t lt = __builtin_amdgcn_readlane(tile, first);
       const bool mine = has && tile == lt;
       const uint64_t peers = __ballot(mine);
-      uint32_t base = 0;
-      if (lane_id() == leader) base = atomicAdd(&counter[lt], __popcll(peers));
-      base = __builtin_amdgcn_readlane(base, leader);
-      if (kFill && mine) {
-        const uint64_t pos = uint64_t(offsets[lt]) + base +
-                             __popcll(peers & lanemask_lt());
-        if (pos < capacity) bins[pos] = uint32_t(t);
+      if (mine) {
+        leader = first;
+        rank = __popcll(peers & lanemask_lt());
+        size = __popcll(peers);
       }
       todo &= ~peers;
+    }
+    // One atomic per distinct tile, all groups' leaders in one instruction,
+    // so the wave waits for a single round trip per k.
+    uint32_t base = 0;
+    if (has && rank == 0) base = atomicAdd(&counter[tile], uint32_t(size));
+    if (kFill) {
+      base = __shfl(base, leader, 64);
+      if (has) {
+        const uint64_t pos = uint64_t(offsets[tile]) + base + uint32_t(rank);
+        if (pos < capacity) bins[pos] = uint32_t(t);
+      }
     }
   }
 }
