@@ -506,7 +506,6 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   constexpr int R = Record<C>::kSize;
   using S = SourceLds<C>;
   __shared__ float4 wave_lds[kBX * kBY / 64][S::kBytes / 16];
-  __shared__ int row_count[kBX * kBY / 64];
   __shared__ int hash_key[kBX * kBY / 64][kHashSlots];
   __shared__ int hash_row[kBX * kBY / 64][kHashSlots];
   const int64_t tile = blockIdx.x;
@@ -523,7 +522,6 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   int* vtx = reinterpret_cast<int*>(wave_lds[wave]) + kRows * R;
   const uint4* stage_surf = reinterpret_cast<const uint4*>(wave_lds[wave]);
   const float4* stage_g = wave_lds[wave] + kPatch;
-  int* nrows = &row_count[wave];
   int* hkey = hash_key[wave];
   int* hrow = hash_row[wave];
   // Nothing to differentiate into, or no gradient scratch to read: the job
@@ -675,7 +673,6 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     const uint64_t act = __ballot(frag);
     if (act == 0) continue;
     for (int i = lane; i < kHashSlots; i += 64) hkey[i] = -1;
-    if (lane == 0) *nrows = 0;
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     __builtin_amdgcn_wave_barrier();
     float* grads = J.d_records;
@@ -695,10 +692,16 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         }
         slot[i] = int(h);
       }
+    }
+    // First occurrences take rows in lane order, vertex by vertex: ballots
+    // rank them, so no LDS counter (and no atomic round trip) is needed.
+    int nr = 0;  // rows taken so far (wave-uniform)
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        if (!first[i]) continue;
-        const int r = atomicAdd(nrows, 1);
+    for (int i = 0; i < 3; ++i) {
+      const bool f = frag && first[i];
+      const uint64_t fm = __ballot(f);
+      if (f) {
+        const int r = nr + __popcll(fm & lanemask_lt());
         hrow[slot[i]] = r < kRows ? r : -1;
         if (r < kRows) {
           vtx[r] = k.v[i];
@@ -708,6 +711,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
           flush_record<C>(grads, k.v[i], rec[i]);
         }
       }
+      nr += __popcll(fm);
     }
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     __builtin_amdgcn_wave_barrier();
@@ -727,7 +731,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     }
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     __builtin_amdgcn_wave_barrier();
-    const int n = min(*nrows, kRows);
+    const int n = min(nr, kRows);
     if (grads) {
       for (int e0 = 0; e0 < n; e0 += 64 / R) {
         const int e = e0 + lane / R, c = lane % R;
