@@ -197,6 +197,7 @@ def cpu_baseline(w, args):
   loss = ((out - tgt) ** 2).sum() / (R * R * 4)
   loss.backward()
   t2 = time.perf_counter()
+  raster_par = cpu_raster_parallel(w, args, ras)
   return dict(value=1.0 / (t2 - t0), unit="renders/s", cores=int(
       torch.get_num_threads()), kind=kind,
               sample=(f"1 view of the workload, fwd+bwd: {kind} CPU "
@@ -204,7 +205,28 @@ def cpu_baseline(w, args):
                       f" + float32 torch-CPU interpolate/splat/over fwd+bwd "
                       f"on {torch.get_num_threads()} threads "
                       f"({(t2 - t1) * 1e3:.0f} ms)"),
-              raster_ms=(t1 - t0) * 1e3, splat_ms=(t2 - t1) * 1e3)
+              raster_ms=(t1 - t0) * 1e3, splat_ms=(t2 - t1) * 1e3,
+              raster_parallel=raster_par)
+
+
+def cpu_raster_parallel(w, args, ras):
+  """The CPU rasterizer on all views at once, one single-threaded image per
+  host thread (the reference kernel is single-threaded per image,
+  rasterize_triangles_impl.cc:37): views/s and the thread count.  ctypes
+  releases the interpreter lock during each call, so the images run
+  concurrently."""
+  from concurrent.futures import ThreadPoolExecutor
+  import torch
+  from oracle import splat_oracle as so
+  R, L = args.resolution, args.layers
+  clips = [so.transform(c, w["v"], torch.float32).numpy() for c in w["cams"]]
+  n = len(clips)
+  threads = max(1, min(n, int(torch.get_num_threads())))
+  with ThreadPoolExecutor(threads) as pool:
+    t0 = time.perf_counter()
+    list(pool.map(lambda c: ras(c, w["t"], R, R, L, 1), clips))
+    dt = time.perf_counter() - t0
+  return dict(views=n, threads=threads, ms=dt * 1e3, views_per_s=n / dt)
 
 
 def finite_differences(r, vertices, target, loss_scale, count=64, eps=1e-3,
@@ -274,9 +296,11 @@ def grad_check(w, args, dev):
     a = a.cpu().double()
     return float((a - b).abs().max() / b.abs().max())
 
-  # eps = 1e-3 is about 0.3 px here: fragments then cross pixel boundaries,
-  # where the splat's 3x3 tap set switches (a small discontinuity), so the
-  # difference quotient also measures those jumps; 1e-4 stays below them.
+  # eps = 1e-3 world units moves a fragment by up to 0.3-0.44 px here, about
+  # 0.6-0.9 sigma of the splat's Gaussian (sigma 0.5 px, splat.py:65-66), so
+  # the central difference carries a large truncation error (it falls as
+  # eps^2: 1e-4 is 0.03 px).  No fragment leaves its pixel cell at either
+  # step (the frozen barycentrics put it at the pixel centre).
   fd = {str(eps): finite_differences(r, w["mesh"]["vertices"],
                                      w["targets"][0], 1.0 / (R * R * 4),
                                      eps=eps) for eps in (1e-3, 1e-4)}
