@@ -246,9 +246,13 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
   const bool in_image = x < W && y < H;
   const int rT = (threadIdx.y + 1) * kRX + threadIdx.x + 1;
   // The 9 surface indices, one signed byte each (tap t in byte t % 4 of
-  // word t / 4): the same bytes the gradient scratch records.
+  // word t / 4): the same bytes the gradient scratch records.  A region
+  // with no fragment (most of a frame) splats nothing and writes no
+  // gradient record, so it needs none.
+  bool region_live = false;
+  for (int l = 0; l < L; ++l) region_live |= layer_live[l] != 0;
   uint32_t sw[3] = {0u, 0u, 0u};
-  if (in_image) {
+  if (in_image && region_live) {
     float zT[kMaxLayers], zS[kMaxLayers];
     for (int l = 0; l < L; ++l) zT[l] = sz[l * R + rT];
 #pragma unroll
@@ -532,6 +536,14 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   const int64_t p = int64_t(y) * W + x;
 
   constexpr int GS = Scratch<C>::kStride, GR = Scratch<C>::kRecord;
+  // A wave whose 64 pixels hold no fragment has nothing to differentiate
+  // (most waves of a frame): it leaves before staging its patch, whose
+  // records it would fetch from memory for nothing.  Waves share no LDS and
+  // no block barrier, so a wave-uniform exit is safe.
+  bool any = false;
+  if (inside)
+    for (int l = 0; l < L; ++l) any |= J.triangle_ids[l * P + p] >= 0;
+  if (__ballot(any) == 0) return;
   // Stage the patch's records (out-of-image targets as zeros).  Records of
   // targets that received no splat weight were never written; they are
   // only read for taps of foreground sources, whose targets all were.
@@ -562,9 +574,6 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
       }
     }
   }
-  bool any = false;
-  if (inside)
-    for (int l = 0; l < L; ++l) any |= J.triangle_ids[l * P + p] >= 0;
   __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
   __builtin_amdgcn_wave_barrier();
 
