@@ -544,6 +544,10 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   if (inside)
     for (int l = 0; l < L; ++l) any |= J.triangle_ids[l * P + p] >= 0;
   if (__ballot(any) == 0) return;
+  // Layer 0's corners are loaded now, so their loads are in flight together
+  // with the patch staging below instead of after it.
+  Corner k0;
+  const bool frag0 = any && corners(J, p, k0);
   // Stage the patch's records (out-of-image targets as zeros).  Records of
   // targets that received no splat weight were never written; they are
   // only read for taps of foreground sources, whose targets all were.
@@ -595,8 +599,8 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   }
   // Every lane of the wave runs every layer iteration (ballots below).
   for (int l = 0; l < L; ++l) {
-    Corner k;
-    const bool frag = any && corners(J, l * P + p, k);
+    Corner k = k0;
+    const bool frag = l == 0 ? frag0 : any && corners(J, l * P + p, k);
     float rec[3][R];
     if (frag) {
       float sx, sy, cx, cy, cw;
