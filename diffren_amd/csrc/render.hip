@@ -198,7 +198,11 @@ __global__ __launch_bounds__(kBX* kBY)
 __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
     const dr_splat_job* __restrict__ jobs, int num_jobs, int L, float coef,
     int grad_mode, float loss_scale, float* __restrict__ loss_partials) {
-  extern __shared__ float lds[];
+  // float4-aligned, so a C % 4 == 0 colour row is read and written as
+  // 16-byte LDS accesses (lanes at a 16-byte stride with 4-byte accesses
+  // meet the same banks several times per instruction).
+  extern __shared__ float4 lds4[];
+  float* lds = reinterpret_cast<float*>(lds4);
   const int R = kRegion;
   int* st = reinterpret_cast<int*>(lds);
   float* sz = lds + L * R;
@@ -321,7 +325,15 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
       sx[i] = px;
       sy[i] = py;
 #pragma unroll
-      for (int c = 0; c < C; ++c) sc[i * C + c] = col[c];
+      for (int c = 0; c < C; ++c) {
+        if constexpr (C % 4 == 0) {
+          if (c % 4 == 0)
+            *reinterpret_cast<float4*>(sc + i * C + c) =
+                make_float4(col[c], col[c + 1], col[c + 2], col[c + 3]);
+        } else {
+          sc[i * C + c] = col[c];
+        }
+      }
     }
     __syncthreads();
     if (in_image) {
@@ -335,9 +347,22 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
         const float w = tap_weight(coef, px, sy[i], u, v, dx, dy);
         const int kc = layer_class(
             int(int8_t(sw[t >> 2] >> (8 * (t & 3)))), l);
+        float cv[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          const float wc = w * sc[i * C + c];
+          if constexpr (C % 4 == 0) {
+            if (c % 4 == 0) {
+              const float4 q =
+                  *reinterpret_cast<const float4*>(sc + i * C + c);
+              cv[c] = q.x; cv[c + 1] = q.y; cv[c + 2] = q.z; cv[c + 3] = q.w;
+            }
+          } else {
+            cv[c] = sc[i * C + c];
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const float wc = w * cv[c];
           acc[0][c] += kc == 0 ? wc : 0.0f;
           acc[1][c] += kc == 1 ? wc : 0.0f;
           acc[2][c] += kc == 2 ? wc : 0.0f;
@@ -499,7 +524,11 @@ struct SourceLds {
   static constexpr bool kStageG = GS <= 8;
   static constexpr int kStageBytes =
       kPatch * 16 + (kStageG ? kPatch * GS * 4 : 0);
-  static constexpr int kRowBytes = kRows * R * 4 + kRows * 4;
+  // Rows are R + 1 floats apart: lanes adding into different rows then
+  // meet different LDS banks (a stride of R = 8 floats puts every lane of a
+  // column on the same few banks).
+  static constexpr int RS = R + 1;
+  static constexpr int kRowBytes = kRows * RS * 4 + kRows * 4;
   static constexpr int kBytes =
       ((kStageBytes > kRowBytes ? kStageBytes : kRowBytes) + 15) & ~15;
 };
@@ -522,8 +551,9 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   // Wave w covers columns 16w .. 16w+15 of the 64x4 tile, all 4 rows.
   const int x = (local % ntx) * kBX + 16 * wave + (lane & 15);
   const int y = (local / ntx) * kBY + (lane >> 4);
-  float(*val)[R] = reinterpret_cast<float(*)[R]>(wave_lds[wave]);
-  int* vtx = reinterpret_cast<int*>(wave_lds[wave]) + kRows * R;
+  constexpr int RS = S::RS;
+  float* val = reinterpret_cast<float*>(wave_lds[wave]);  // [kRows][RS]
+  int* vtx = reinterpret_cast<int*>(wave_lds[wave]) + kRows * RS;
   const uint4* stage_surf = reinterpret_cast<const uint4*>(wave_lds[wave]);
   const float4* stage_g = wave_lds[wave] + kPatch;
   int* hkey = hash_key[wave];
@@ -719,7 +749,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         if (r < kRows) {
           vtx[r] = k.v[i];
 #pragma unroll
-          for (int c = 0; c < R; ++c) val[r][c] = rec[i][c];
+          for (int c = 0; c < R; ++c) val[r * RS + c] = rec[i][c];
         } else {
           flush_record<C>(grads, k.v[i], rec[i]);
         }
@@ -739,7 +769,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         }
 #pragma unroll
         for (int c = 0; c < R; ++c)
-          if (rec[i][c] != 0.0f) atomicAdd(&val[r][c], rec[i][c]);
+          if (rec[i][c] != 0.0f) atomicAdd(&val[r * RS + c], rec[i][c]);
       }
     }
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
@@ -749,7 +779,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
       for (int e0 = 0; e0 < n; e0 += 64 / R) {
         const int e = e0 + lane / R, c = lane % R;
         if (e < n) {
-          const float g = val[e][c];
+          const float g = val[e * RS + c];
           if (g != 0.0f) unsafeAtomicAdd(grads + int64_t(vtx[e]) * R + c, g);
         }
       }
