@@ -722,18 +722,28 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     int slot[3] = {0, 0, 0};
     bool first[3] = {false, false, false};
     if (frag) {
+      // Linear probing; the table has more slots than a wave has entries,
+      // so every probe sequence ends at a free slot or a match.  The three
+      // vertices' first probes are issued back to back and collisions walk
+      // on afterwards: slots are never freed, so each vertex still ends in
+      // exactly one slot whatever the order of the probes.
+      uint32_t h[3];
+      int old[3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        // Linear probing; the table has more slots than a wave has entries,
-        // so every probe sequence ends at a free slot or a match.
-        uint32_t h = (uint32_t(k.v[i]) * 2654435761u) >> 24;
-        for (int probe = 0; probe < kHashSlots; ++probe) {
-          const int old = atomicCAS(&hkey[h], -1, k.v[i]);
-          if (old == -1) { first[i] = true; break; }
-          if (old == k.v[i]) break;
-          h = (h + 1) & (kHashSlots - 1);
+        h[i] = (uint32_t(k.v[i]) * 2654435761u) >> 24;
+        old[i] = atomicCAS(&hkey[h[i]], -1, k.v[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        for (int probe = 1;
+             probe < kHashSlots && old[i] != -1 && old[i] != k.v[i];
+             ++probe) {
+          h[i] = (h[i] + 1) & (kHashSlots - 1);
+          old[i] = atomicCAS(&hkey[h[i]], -1, k.v[i]);
         }
-        slot[i] = int(h);
+        first[i] = old[i] == -1;
+        slot[i] = int(h[i]);
       }
     }
     // First occurrences take rows in lane order, vertex by vertex: ballots
