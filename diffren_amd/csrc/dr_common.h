@@ -53,6 +53,23 @@ inline int fail(int code, const std::string& msg) {
 // Wave (64 lanes on CDNA) helpers.
 namespace dr {
 __device__ inline int lane_id() { return __lane_id(); }
+
+// Screen tile of this block for tile-per-block kernels, XCD-aware.  The
+// dispatcher deals workgroups to the chip's 8 XCDs round-robin (block b to
+// XCD b % 8), and each XCD has its own L2, so consecutive tiles -- which
+// share vertices, triangles and halo pixels -- would land in 8 different
+// L2s.  This maps each XCD's blocks onto runs of kRun consecutive tiles
+// (fine-grained, so the XCDs' loads stay balanced): block 8k + x takes tile
+// ((k / kRun) * 8 + x) * kRun + k % kRun.  A bijection on the largest
+// multiple of 8 kRun blocks; the remaining tail keeps its own index.
+template <int kRun>
+__device__ inline int64_t xcd_tile() {
+  const int64_t b = blockIdx.x, n = gridDim.x;
+  const int64_t full = n / (8 * kRun) * (8 * kRun);
+  if (b >= full) return b;
+  const int64_t x = b % 8, k = b / 8;
+  return ((k / kRun) * 8 + x) * kRun + k % kRun;
+}
 __device__ inline uint64_t lanemask_lt() {
   return (uint64_t(1) << lane_id()) - 1;
 }
