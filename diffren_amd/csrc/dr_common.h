@@ -54,16 +54,16 @@ inline int fail(int code, const std::string& msg) {
 namespace dr {
 __device__ inline int lane_id() { return __lane_id(); }
 
-// Screen tile of this block for tile-per-block kernels, XCD-aware.  The
-// dispatcher deals workgroups to the chip's 8 XCDs round-robin (block b to
-// XCD b % 8), and each XCD has its own L2, so consecutive tiles -- which
+// Work item (screen tile, or chunk of triangles) of this block, XCD-aware.
+// The dispatcher deals workgroups to the chip's 8 XCDs round-robin (block b
+// to XCD b % 8), and each XCD has its own L2, so consecutive items -- which
 // share vertices, triangles and halo pixels -- would land in 8 different
-// L2s.  This maps each XCD's blocks onto runs of kRun consecutive tiles
-// (fine-grained, so the XCDs' loads stay balanced): block 8k + x takes tile
+// L2s.  This maps each XCD's blocks onto runs of kRun consecutive items
+// (fine-grained, so the XCDs' loads stay balanced): block 8k + x takes item
 // ((k / kRun) * 8 + x) * kRun + k % kRun.  A bijection on the largest
 // multiple of 8 kRun blocks; the remaining tail keeps its own index.
 template <int kRun>
-__device__ inline int64_t xcd_tile() {
+__device__ inline int64_t xcd_block() {
   const int64_t b = blockIdx.x, n = gridDim.x;
   const int64_t full = n / (8 * kRun) * (8 * kRun);
   if (b >= full) return b;

@@ -672,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void raster_tile_kernel(
   __shared__ uint64_t cand[kMaxCandidates];
   __shared__ int cand_n;
 
-  const int64_t flat_tile = xcd_tile<4>();
+  const int64_t flat_tile = xcd_block<4>();
   const int j = find_job<true>(J, flat_tile);
   const dr_raster_job& job = J.table[j];
   const int W = job.width, H = job.height;

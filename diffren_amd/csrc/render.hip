@@ -211,7 +211,7 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
   float* sc = sy + R;  // [R][C]
   __shared__ int layer_live[kMaxLayers];
 
-  const int64_t tile = xcd_tile<16>();
+  const int64_t tile = xcd_block<16>();
   const dr_splat_job& J = jobs[find_job(jobs, num_jobs, tile)];
   const int W = J.width, H = J.height;
   const int ntx = (W + kBX - 1) / kBX;
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   __shared__ float4 wave_lds[kBX * kBY / 64][S::kBytes / 16];
   __shared__ int hash_key[kBX * kBY / 64][kHashSlots];
   __shared__ int hash_row[kBX * kBY / 64][kHashSlots];
-  const int64_t tile = xcd_tile<32>();
+  const int64_t tile = xcd_block<32>();
   const dr_splat_job& J = jobs[find_job(jobs, num_jobs, tile)];
   const int W = J.width, H = J.height;
   const int ntx = (W + kBX - 1) / kBX;
