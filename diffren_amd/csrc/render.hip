@@ -46,15 +46,31 @@ constexpr int kRX = kBX + 2, kRY = kBY + 2, kRegion = kRX * kRY;
 constexpr int kMaxLayers = 8;
 constexpr int kMaxChannels = 16;
 
-// Per-pixel gradient scratch written by the target pass for the source pass:
-// 3 classes x kStride floats (d acc colour [C], d acc weight, padding to a
-// float4 multiple), then 16 bytes holding the pixel's 9 surface indices
-// (one signed byte per tap, tap order t = 3 (u + 1) + (v + 1)).
+// Per-pixel gradient scratch written by the target pass for the source pass,
+// kRecord floats per pixel in two planes of the job's buffer:
+//  * hot plane [P][kHot]: 16 bytes holding the pixel's 9 surface indices
+//    (one signed byte per tap, tap order t = 3 (u + 1) + (v + 1)), then the
+//    "surface" class (1) -- what nearly every fragment reads;
+//  * cold plane [P][kCold] after it: classes 0 and 2, written only where the
+//    class received splat weight (only then does a source read it).
+// A class is kStride floats: d acc colour [C], d acc weight, padding to a
+// float4 multiple.
 template <int C>
 struct Scratch {
   static constexpr int kStride = ((C + 1) + 3) & ~3;
-  static constexpr int kRecord = 3 * kStride + 4;  // floats
+  static constexpr int kHot = 4 + kStride;
+  static constexpr int kCold = 2 * kStride;
+  static constexpr int kRecord = kHot + kCold;  // floats
 };
+
+// Class k of pixel p's record in a job's scratch of P pixels.
+template <int C>
+__device__ __forceinline__ float* scratch_class(float* base, int64_t P,
+                                                int64_t p, int k) {
+  using S = Scratch<C>;
+  return k == 1 ? base + p * S::kHot + 4
+                : base + P * S::kHot + p * S::kCold + (k == 0 ? 0 : S::kStride);
+}
 
 __device__ __forceinline__ int find_job(const dr_splat_job* jobs, int n,
                                         int64_t tile) {
@@ -420,9 +436,10 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
       dn[0][C - 1] += da0;
       dn[1][C - 1] += da1;
       constexpr int GS = Scratch<C>::kStride;
-      float* gp = J.grad_scratch + p * Scratch<C>::kRecord;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
+        if (k != 1 && !(acc[k][C] > 0.0f)) continue;  // never read
+        float* gp = scratch_class<C>(J.grad_scratch, P, p, k);
         const float ds = acc[k][C] > 1.0f ? -(s[k] * s[k]) : 0.0f;
         float rec[GS];
         float dw = 0.0f;
@@ -436,10 +453,10 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
         for (int c = C + 1; c < GS; ++c) rec[c] = 0.0f;
 #pragma unroll
         for (int c = 0; c < GS; c += 4)
-          *reinterpret_cast<float4*>(gp + k * GS + c) =
+          *reinterpret_cast<float4*>(gp + c) =
               make_float4(rec[c], rec[c + 1], rec[c + 2], rec[c + 3]);
       }
-      *reinterpret_cast<uint4*>(gp + 3 * GS) =
+      *reinterpret_cast<uint4*>(J.grad_scratch + p * Scratch<C>::kHot) =
           make_uint4(sw[0], sw[1], sw[2], 0u);
     }
   }
@@ -565,7 +582,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   const int64_t P = int64_t(W) * H;
   const int64_t p = int64_t(y) * W + x;
 
-  constexpr int GS = Scratch<C>::kStride, GR = Scratch<C>::kRecord;
+  constexpr int GS = Scratch<C>::kStride;
   // A wave whose 64 pixels hold no fragment has nothing to differentiate
   // (most waves of a frame): it leaves before staging its patch, whose
   // records it would fetch from memory for nothing.  Waves share no LDS and
@@ -593,12 +610,13 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
       for (int i = 0; i < (S::kStageG ? GS / 4 : 1); ++i)
         g4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       if (tx >= 0 && tx < W && ty >= 0 && ty < H) {
-        const float* gp = J.grad_scratch + (int64_t(ty) * W + tx) * GR;
-        sv = *reinterpret_cast<const uint4*>(gp + 3 * GS);
+        const float* gp =
+            J.grad_scratch + (int64_t(ty) * W + tx) * Scratch<C>::kHot;
+        sv = *reinterpret_cast<const uint4*>(gp);
         if (S::kStageG) {
 #pragma unroll
           for (int i = 0; i < GS / 4; ++i)
-            g4[i] = *reinterpret_cast<const float4*>(gp + GS + 4 * i);
+            g4[i] = *reinterpret_cast<const float4*>(gp + 4 + 4 * i);
         }
       }
       ss[e] = sv;
@@ -664,8 +682,8 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
             gv[c + 3] = q4.w;
           }
         } else {
-          const float* gp =
-              J.grad_scratch + (int64_t(ty) * W + tx) * GR + cls * GS;
+          const float* gp = scratch_class<C>(J.grad_scratch, P,
+                                             int64_t(ty) * W + tx, cls);
 #pragma unroll
           for (int c = 0; c < GS; c += 4) {
             const float4 q4 = *reinterpret_cast<const float4*>(gp + c);
