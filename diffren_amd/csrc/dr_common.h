@@ -52,6 +52,21 @@ inline int fail(int code, const std::string& msg) {
 
 // Wave (64 lanes on CDNA) helpers.
 namespace dr {
+
+// Load through a global-address-space pointer: pointers read out of job
+// tables are generic, and generic (flat) loads also count against the LDS
+// wait counter, so a wait for LDS results would wait for them too.
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)(p);
+}
+
+// 16-byte global load of 4 floats at p (16-byte aligned).
+__device__ __forceinline__ float4 ldg4(const float* p) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4 q = *(const __attribute__((address_space(1))) f4*)(p);
+  return make_float4(q.x, q.y, q.z, q.w);
+}
 __device__ inline int lane_id() { return __lane_id(); }
 
 // Work item (screen tile, or chunk of triangles) of this block, XCD-aware.

@@ -288,20 +288,28 @@ __device__ __forceinline__ float4 transform_point(const float* __restrict__ M,
 }
 
 // Loads triangle `t` of `job` in clip space; false if an index is out of
-// range.
+// range.  The three indices are loaded together, then the three vertices
+// together (two dependent round trips, not one per vertex), as global-memory
+// loads: the job table hands out generic pointers, whose flat loads would
+// also hold up every wait on the LDS counter.
 __device__ __forceinline__ bool load_triangle(const dr_raster_job& job,
                                               int t, float4 v[3]) {
-  const int32_t* tri = job.triangles + 3 * int64_t(t);
+  const int64_t b = 3 * int64_t(t);
+  const int32_t i0 = ldg(job.triangles + b), i1 = ldg(job.triangles + b + 1),
+                i2 = ldg(job.triangles + b + 2);
+  const uint32_t nv = uint32_t(job.num_vertices);
+  if (uint32_t(i0) >= nv || uint32_t(i1) >= nv || uint32_t(i2) >= nv)
+    return false;
+  const int32_t idx[3] = {i0, i1, i2};
+  if (job.camera) {
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const int32_t i = tri[k];
-    if (i < 0 || i >= job.num_vertices) return false;
-    if (job.camera) {
-      const float* p = job.vertices + 3 * int64_t(i);
+    for (int k = 0; k < 3; ++k) {
+      const float* p = job.vertices + 3 * int64_t(idx[k]);
       v[k] = transform_point(job.camera, p[0], p[1], p[2]);
-    } else {
-      v[k] = reinterpret_cast<const float4*>(job.vertices)[i];
     }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = ldg4(job.vertices + 4 * int64_t(idx[k]));
   }
   return true;
 }
@@ -659,8 +667,12 @@ struct TriRect {
 };
 
 // Rasterizes all bin entries of one tile per block, then resolves the tile.
+// Five waves per SIMD (at most 96 VGPRs): left to itself the compiler
+// keeps more loads in flight per wave at 97-100 VGPRs, four waves per SIMD,
+// which is slower.
 template <int TW, int TH>
-__global__ __launch_bounds__(kBlock) void raster_tile_kernel(
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
     Jobs J, int L, int cull, const uint32_t* __restrict__ counts,
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ bins,
     uint64_t capacity, bool sparse) {

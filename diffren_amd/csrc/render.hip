@@ -88,17 +88,20 @@ struct Corner {
 };
 
 // Triangle corners of pixel-layer q; false on background or bad indices.
+// The three indices and three barycentrics are loaded together (global-
+// memory loads; see ldg) before any of them is checked.
 __device__ __forceinline__ bool corners(const dr_splat_job& J, int64_t q,
                                         Corner& k) {
-  const int t = J.triangle_ids[q];
+  const int t = ldg(J.triangle_ids + q);
   if (t < 0 || t >= J.num_triangles) return false;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    k.v[i] = J.triangles[3 * int64_t(t) + i];
-    if (unsigned(k.v[i]) >= unsigned(J.num_vertices)) return false;
-    k.b[i] = J.barycentrics[3 * q + i];
+    k.v[i] = ldg(J.triangles + 3 * int64_t(t) + i);
+    k.b[i] = ldg(J.barycentrics + 3 * q + i);
   }
-  return true;
+  const unsigned nv = unsigned(J.num_vertices);
+  return unsigned(k.v[0]) < nv && unsigned(k.v[1]) < nv &&
+         unsigned(k.v[2]) < nv;
 }
 
 __device__ __forceinline__ float interp3(const Corner& k, float a0, float a1,
