@@ -61,11 +61,24 @@ __device__ __forceinline__ T ldg(const T* p) {
   return *(const __attribute__((address_space(1))) T*)(p);
 }
 
-// 16-byte global load of 4 floats at p (16-byte aligned).
+// 16-byte global loads at p (16-byte aligned).
 __device__ __forceinline__ float4 ldg4(const float* p) {
   typedef float f4 __attribute__((ext_vector_type(4)));
   const f4 q = *(const __attribute__((address_space(1))) f4*)(p);
   return make_float4(q.x, q.y, q.z, q.w);
+}
+__device__ __forceinline__ uint4 ldgu4(const uint32_t* p) {
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  const u4 q = *(const __attribute__((address_space(1))) u4*)(p);
+  return make_uint4(q.x, q.y, q.z, q.w);
+}
+
+// Float atomic add through a global-address-space pointer (hardware
+// global_atomic_add_f32, relaxed, device scope; a generic pointer would make
+// it a flat atomic, which also counts against the LDS wait counter).
+__device__ __forceinline__ void gatomic_add(float* p, float v) {
+  __hip_atomic_fetch_add((__attribute__((address_space(1))) float*)(p), v,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline int lane_id() { return __lane_id(); }
 

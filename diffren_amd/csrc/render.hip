@@ -135,7 +135,7 @@ __device__ __forceinline__ float center_offset(float c, float w, int size) {
 // (the splat takes z from the rasterizer's z buffer).
 __device__ __forceinline__ float4 clip_vertex(const dr_splat_job& J, int v) {
   if (J.clip_vertices)
-    return reinterpret_cast<const float4*>(J.clip_vertices)[v];
+    return ldg4(J.clip_vertices + 4 * int64_t(v));
   const float* p = J.positions + 3 * int64_t(v);
   const float x = p[0], y = p[1], z = p[2];
   const float* M = J.camera;
@@ -168,7 +168,7 @@ __device__ __forceinline__ float viewport_z(float z) {
 // buffer is read only under a triangle (the rasterizer's sparse mode leaves
 // background z unwritten).
 __device__ __forceinline__ float layer_z(const dr_splat_job& J, int64_t q) {
-  return J.triangle_ids[q] >= 0 ? viewport_z(J.z_buffer[q]) : 1.0f;
+  return ldg(J.triangle_ids + q) >= 0 ? viewport_z(ldg(J.z_buffer + q)) : 1.0f;
 }
 
 __device__ __forceinline__ int surface_index(const float* zS, const float* zT,
@@ -255,8 +255,8 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
     float z = 1.0f;
     if (px >= 0 && px < W && py >= 0 && py < H) {
       const int64_t q = l * P + int64_t(py) * W + px;
-      t = J.triangle_ids[q];
-      if (t >= 0 && t < J.num_triangles) z = viewport_z(J.z_buffer[q]);
+      t = ldg(J.triangle_ids + q);
+      if (t >= 0 && t < J.num_triangles) z = viewport_z(ldg(J.z_buffer + q));
       else t = -1;
     }
     st[i] = t;
@@ -313,8 +313,8 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
           ok[j] = true;
 #pragma unroll
           for (int e = 0; e < 3; ++e) {
-            k[j].v[e] = J.triangles[3 * int64_t(t) + e];
-            k[j].b[e] = J.barycentrics[3 * q + e];
+            k[j].v[e] = ldg(J.triangles + 3 * int64_t(t) + e);
+            k[j].b[e] = ldg(J.barycentrics + 3 * q + e);
           }
         }
       }
@@ -339,7 +339,7 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
         const float* a2 = J.colors + int64_t(k[j].v[2]) * C;
 #pragma unroll
         for (int c = 0; c < C; ++c)
-          col[c] = interp3(k[j], a0[c], a1[c], a2[c]);
+          col[c] = interp3(k[j], ldg(a0 + c), ldg(a1 + c), ldg(a2 + c));
       }
       sx[i] = px;
       sy[i] = py;
@@ -413,11 +413,11 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
       o = n[0][c] + (1.0f - a0) * o;
       if (J.image) J.image[p * C + c] = o;
       if (mode == 1) {
-        const float d = o - J.target[p * C + c];
+        const float d = o - ldg(J.target + p * C + c);
         partial += d * d;
         g[c] = (2.0f * d) * loss_scale;
       } else if (mode == 2) {
-        g[c] = J.d_image[p * C + c];
+        g[c] = ldg(J.d_image + p * C + c);
       }
     }
     // A pixel that received no splat weight is never read by the source
@@ -526,7 +526,7 @@ __device__ __forceinline__ void flush_record(float* grads, int v,
   if (grads == nullptr) return;
 #pragma unroll
   for (int c = 0; c < R; ++c)
-    if (rec[c] != 0.0f) unsafeAtomicAdd(grads + int64_t(v) * R + c, rec[c]);
+    if (rec[c] != 0.0f) gatomic_add(grads + int64_t(v) * R + c, rec[c]);
 }
 
 // The wave's 16x4 sources splat onto an 18x6 patch of targets.  Their
@@ -592,7 +592,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   // no block barrier, so a wave-uniform exit is safe.
   bool any = false;
   if (inside)
-    for (int l = 0; l < L; ++l) any |= J.triangle_ids[l * P + p] >= 0;
+    for (int l = 0; l < L; ++l) any |= ldg(J.triangle_ids + l * P + p) >= 0;
   if (__ballot(any) == 0) return;
   // Layer 0's corners are loaded now, so their loads are in flight together
   // with the patch staging below instead of after it.
@@ -615,11 +615,11 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
       if (tx >= 0 && tx < W && ty >= 0 && ty < H) {
         const float* gp =
             J.grad_scratch + (int64_t(ty) * W + tx) * Scratch<C>::kHot;
-        sv = *reinterpret_cast<const uint4*>(gp);
+        sv = ldgu4(reinterpret_cast<const uint32_t*>(gp));
         if (S::kStageG) {
 #pragma unroll
           for (int i = 0; i < GS / 4; ++i)
-            g4[i] = *reinterpret_cast<const float4*>(gp + 4 + 4 * i);
+            g4[i] = ldg4(gp + 4 + 4 * i);
         }
       }
       ss[e] = sv;
@@ -662,7 +662,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
       float col[C], dcol[C];
 #pragma unroll
       for (int c = 0; c < C; ++c) {
-        col[c] = interp3(k, a0[c], a1[c], a2[c]);
+        col[c] = interp3(k, ldg(a0 + c), ldg(a1 + c), ldg(a2 + c));
         dcol[c] = 0.0f;
       }
       float gx = 0.0f, gy = 0.0f;
@@ -689,7 +689,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
                                              int64_t(ty) * W + tx, cls);
 #pragma unroll
           for (int c = 0; c < GS; c += 4) {
-            const float4 q4 = *reinterpret_cast<const float4*>(gp + c);
+            const float4 q4 = ldg4(gp + c);
             gv[c] = q4.x; gv[c + 1] = q4.y; gv[c + 2] = q4.z;
             gv[c + 3] = q4.w;
           }
@@ -811,7 +811,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         const int e = e0 + lane / R, c = lane % R;
         if (e < n) {
           const float g = val[e * RS + c];
-          if (g != 0.0f) unsafeAtomicAdd(grads + int64_t(vtx[e]) * R + c, g);
+          if (g != 0.0f) gatomic_add(grads + int64_t(vtx[e]) * R + c, g);
         }
       }
     }
