@@ -73,6 +73,20 @@ __device__ __forceinline__ uint4 ldgu4(const uint32_t* p) {
   return make_uint4(q.x, q.y, q.z, q.w);
 }
 
+// *p for a wave-uniform p, word by word through readfirstlane, so the value
+// lives in scalar registers.
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+  T out;
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+  uint32_t* dst = reinterpret_cast<uint32_t*>(&out);
+#pragma unroll
+  for (int i = 0; i < int(sizeof(T) / 4); ++i)
+    dst[i] = __builtin_amdgcn_readfirstlane(ldg(src + i));
+  return out;
+}
+
 // Float atomic add through a global-address-space pointer (hardware
 // global_atomic_add_f32, relaxed, device scope; a generic pointer would make
 // it a flat atomic, which also counts against the LDS wait counter).

@@ -685,8 +685,10 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
   __shared__ int cand_n;
 
   const int64_t flat_tile = xcd_block<4>();
-  const int j = find_job<true>(J, flat_tile);
-  const dr_raster_job& job = J.table[j];
+  // One job per block: its fields are held in scalar registers (loaded
+  // once, made wave-uniform), not re-gathered per lane at every triangle.
+  const int j = __builtin_amdgcn_readfirstlane(find_job<true>(J, flat_tile));
+  const dr_raster_job job = load_uniform(J.table + j);
   const int W = job.width, H = job.height;
   const int ntx = tiles_x(job, TW);
   const int local = int(flat_tile - job.tile_begin);
