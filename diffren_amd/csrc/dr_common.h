@@ -105,12 +105,24 @@ __device__ inline int lane_id() { return __lane_id(); }
 // ((k / kRun) * 8 + x) * kRun + k % kRun.  A bijection on the largest
 // multiple of 8 kRun blocks; the remaining tail keeps its own index.
 template <int kRun>
-__device__ inline int64_t xcd_block() {
-  const int64_t b = blockIdx.x, n = gridDim.x;
+__device__ inline int64_t xcd_order(int64_t b, int64_t n) {
   const int64_t full = n / (8 * kRun) * (8 * kRun);
   if (b >= full) return b;
   const int64_t x = b % 8, k = b / 8;
   return ((k / kRun) * 8 + x) * kRun + k % kRun;
+}
+template <int kRun>
+__device__ inline int64_t xcd_block() {
+  return xcd_order<kRun>(blockIdx.x, gridDim.x);
+}
+// The same for a 2-D grid of tiles, dispatched in row-major block order:
+// the (x, y) block this workgroup takes.
+template <int kRun>
+__device__ inline int2 xcd_block_2d() {
+  const int64_t w = gridDim.x;
+  const int64_t l = xcd_order<kRun>(int64_t(blockIdx.y) * w + blockIdx.x,
+                                    w * gridDim.y);
+  return make_int2(int(l % w), int(l / w));
 }
 __device__ inline uint64_t lanemask_lt() {
   return (uint64_t(1) << lane_id()) - 1;

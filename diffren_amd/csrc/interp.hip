@@ -73,25 +73,51 @@ __global__ __launch_bounds__(kBlock) void interp_fwd_kernel(
   }
 }
 
+// When C divides 64 (C = 1, 2, 4, 8, 16, 32, 64) the wave's 64 x 3
+// (pixel, vertex) records are flushed C lanes per record: in round r the
+// lanes take records r * (64 / C) ... in lane order, each lane one channel,
+// so one atomic instruction touches 64 / C contiguous C-float rows instead
+// of 64 scattered addresses (grouped float atomics run several times faster
+// than scattered ones).  Same atomics, same values; only their grouping
+// differs.  Other C: one pixel per lane.
 __global__ __launch_bounds__(kBlock) void interp_bwd_kernel(
     const int32_t* __restrict__ ids, const float* __restrict__ barys,
     const int32_t* __restrict__ tris, int32_t T, const float* __restrict__ dout,
     int32_t V, int32_t C, int64_t N, float* __restrict__ dA) {
   const int64_t p = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (p >= N) return;
-  int v0, v1, v2;
-  if (!pixel_vertices(ids, tris, T, V, p, v0, v1, v2)) return;
-  const float b0 = barys[3 * p], b1 = barys[3 * p + 1], b2 = barys[3 * p + 2];
+  int v[3] = {0, 0, 0};
+  float b[3] = {0.0f, 0.0f, 0.0f};
+  const bool ok = p < N && pixel_vertices(ids, tris, T, V, p, v[0], v[1], v[2]);
+  if (ok) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) b[k] = barys[3 * p + k];
+  }
+  if (64 % C == 0) {
+    // Whole waves take part (no lane has left): the shuffles read every lane.
+    const int lane = __lane_id();
+    const int per = 64 / C, c = lane % C;
+    const int64_t wave_base = p - lane;
+    for (int r = 0; r < 3 * C; ++r) {
+      const int pair = r * per + lane / C;  // record: vertex k of pixel src
+      const int k = (r * per) >> 6;         // the same for the whole wave
+      const int src = pair & 63;
+      const int vk = __shfl(k == 0 ? v[0] : (k == 1 ? v[1] : v[2]), src);
+      const float bk = __shfl(k == 0 ? b[0] : (k == 1 ? b[1] : b[2]), src);
+      if (!__shfl(int(ok), src)) continue;
+      const float gc = dout[(wave_base + src) * C + c];
+      if (gc == 0.0f) continue;
+      gatomic_add(dA + int64_t(vk) * C + c, bk * gc);
+    }
+    return;
+  }
+  if (!ok) return;
   const float* g = dout + p * C;
-  float* d0 = dA + int64_t(v0) * C;
-  float* d1 = dA + int64_t(v1) * C;
-  float* d2 = dA + int64_t(v2) * C;
   for (int c = 0; c < C; ++c) {
     const float gc = g[c];
     if (gc == 0.0f) continue;
-    unsafeAtomicAdd(d0 + c, b0 * gc);
-    unsafeAtomicAdd(d1 + c, b1 * gc);
-    unsafeAtomicAdd(d2 + c, b2 * gc);
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      gatomic_add(dA + int64_t(v[k]) * C + c, b[k] * gc);
   }
 }
 

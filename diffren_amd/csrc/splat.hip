@@ -139,8 +139,9 @@ template <int C>
 __global__ __launch_bounds__(kBX* kBY) void splat_fwd_kernel(
     SplatFrame f, float* __restrict__ out, float* __restrict__ accum,
     float* __restrict__ norm, float* __restrict__ accum_w) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
+  const int2 tb = xcd_block_2d<16>();
+  const int x = tb.x * kBX + threadIdx.x;
+  const int y = tb.y * kBY + threadIdx.y;
   if (x >= f.W || y >= f.H) return;
   float acc[3][C + 1];
   accumulate_target<C>(f, x, y, acc);
@@ -177,8 +178,9 @@ __global__ __launch_bounds__(kBX* kBY) void splat_fwd_kernel(
 template <int C>
 __global__ __launch_bounds__(kBX* kBY) void splat_bwd_target_kernel(
     SplatFrame f, const float* __restrict__ d_out, float* __restrict__ G) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
+  const int2 tb = xcd_block_2d<16>();
+  const int x = tb.x * kBX + threadIdx.x;
+  const int y = tb.y * kBY + threadIdx.y;
   if (x >= f.W || y >= f.H) return;
   float acc[3][C + 1];
   accumulate_target<C>(f, x, y, acc);
@@ -226,8 +228,9 @@ template <int C>
 __global__ __launch_bounds__(kBX* kBY) void splat_bwd_source_kernel(
     SplatFrame f, const float* __restrict__ G, float* __restrict__ d_color,
     float* __restrict__ d_xyz) {
-  const int x = blockIdx.x * kBX + threadIdx.x;
-  const int y = blockIdx.y * kBY + threadIdx.y;
+  const int2 tb = xcd_block_2d<16>();
+  const int x = tb.x * kBX + threadIdx.x;
+  const int y = tb.y * kBY + threadIdx.y;
   if (x >= f.W || y >= f.H) return;
   const int64_t P = int64_t(f.H) * f.W;
   const int64_t p = int64_t(y) * f.W + x;
