@@ -215,7 +215,11 @@ __device__ __forceinline__ bool shade_pixel(int ix, int iy, float sx2,
                                             const float4 v[3], Fragment& f) {
   const float px = sx2 * (static_cast<float>(ix) + 0.5f) - 1.0f;
   const float py = sy2 * (static_cast<float>(iy) + 0.5f) - 1.0f;
-  const int64_t pxi = to_fixed(px), pyi = to_fixed(py);
+  // Pixel centres of the image lie in (-1, 1), so px * 2^16 is within 32-bit
+  // range and its 32-bit truncation is to_fixed's value exactly; the
+  // sign-extended 32-bit operand makes the edge products cheaper.
+  const int64_t pxi = static_cast<int32_t>(px * 65536.0f);
+  const int64_t pyi = static_cast<int32_t>(py * 65536.0f);
   int64_t e[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
