@@ -162,6 +162,32 @@ __device__ __forceinline__ void adjugate(const int64_t m[9], int cull,
   }
 }
 
+// Fixed-point matrix and adjugate of a triangle.  When every coordinate is
+// below 2^15 in magnitude (the common case; NaN fails the test) its
+// fixed-point value fits in 32 bits and equals to_fixed's, and the
+// adjugate's products are formed from sign-extended 32-bit operands, which
+// are cheaper; the wrapping 64-bit arithmetic is the same either way.
+__device__ __forceinline__ void edge_setup(const float4 v[3], int cull,
+                                           int64_t r[9]) {
+  const float lim = 32768.0f;
+  const bool small =
+      fabsf(v[0].x) < lim && fabsf(v[1].x) < lim && fabsf(v[2].x) < lim &&
+      fabsf(v[0].y) < lim && fabsf(v[1].y) < lim && fabsf(v[2].y) < lim &&
+      fabsf(v[0].w) < lim && fabsf(v[1].w) < lim && fabsf(v[2].w) < lim;
+  int64_t m[9];
+  if (small) {
+    const float c[9] = {v[0].x, v[1].x, v[2].x, v[0].y, v[1].y,
+                        v[2].y, v[0].w, v[1].w, v[2].w};
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      m[i] = static_cast<int32_t>(c[i] * 65536.0f);
+    adjugate(m, cull, r);
+  } else {
+    fixed_matrix(v, m);
+    adjugate(m, cull, r);
+  }
+}
+
 // Conservative early rejection of triangles that cannot cover any pixel
 // because of culling.  Exact: it only fires when every fixed-point entry is
 // below 2^20 in magnitude (so det and all edge values are computed without
@@ -575,9 +601,8 @@ __device__ __forceinline__ bool fragment_at(const dr_raster_job& job, int t,
     return false;
   if (ix < b.left || ix >= b.right || iy < b.bottom || iy >= b.top)
     return false;
-  int64_t m[9], r[9];
-  fixed_matrix(v, m);
-  adjugate(m, cull, r);
+  int64_t r[9];
+  edge_setup(v, cull, r);
   return shade_pixel(ix, iy, sx2, sy2, r, v, f);
 }
 
@@ -749,9 +774,8 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
         if (area > kWaveRasterArea) {
           wave_job = true;
         } else if (area > 0) {
-          int64_t m[9], r[9];
-          fixed_matrix(v, m);
-          adjugate(m, cull, r);
+          int64_t r[9];
+          edge_setup(v, cull, r);
           for (int iy = rect.y0; iy < rect.y1; ++iy)
             for (int ix = rect.x0; ix < rect.x1; ++ix) {
               Fragment f;
@@ -778,9 +802,8 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
       const TriRect rq = queue[q];
       float4 v[3];
       load_triangle(job, rq.tri, v);
-      int64_t m[9], r[9];
-      fixed_matrix(v, m);
-      adjugate(m, cull, r);
+      int64_t r[9];
+      edge_setup(v, cull, r);
       const int rw = rq.x1 - rq.x0;
       const int area = rw * (rq.y1 - rq.y0);
       for (int k = lane_id(); k < area; k += 64) {
@@ -823,9 +846,8 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
         id = int(uint32_t(key));
         float4 v[3];
         load_triangle(job, id, v);
-        int64_t m[9], r[9];
-        fixed_matrix(v, m);
-        adjugate(m, cull, r);
+        int64_t r[9];
+        edge_setup(v, cull, r);
         shade_pixel(ix, iy, sx2, sy2, r, v, f);
       }
       job.triangle_ids[o] = id;
