@@ -167,20 +167,27 @@ __device__ __forceinline__ void adjugate(const int64_t m[9], int cull,
 // fixed-point value fits in 32 bits and equals to_fixed's, and the
 // adjugate's products are formed from sign-extended 32-bit operands, which
 // are cheaper; the wrapping 64-bit arithmetic is the same either way.
+__device__ __forceinline__ bool small_coords(const float4 v[3]) {
+  const float lim = 32768.0f;
+  return fabsf(v[0].x) < lim && fabsf(v[1].x) < lim && fabsf(v[2].x) < lim &&
+         fabsf(v[0].y) < lim && fabsf(v[1].y) < lim && fabsf(v[2].y) < lim &&
+         fabsf(v[0].w) < lim && fabsf(v[1].w) < lim && fabsf(v[2].w) < lim;
+}
+
+// fixed_matrix for small_coords triangles: 32-bit conversions.
+__device__ __forceinline__ void fixed_matrix_small(const float4 v[3],
+                                                   int64_t m[9]) {
+  const float c[9] = {v[0].x, v[1].x, v[2].x, v[0].y, v[1].y,
+                      v[2].y, v[0].w, v[1].w, v[2].w};
+#pragma unroll
+  for (int i = 0; i < 9; ++i) m[i] = static_cast<int32_t>(c[i] * 65536.0f);
+}
+
 __device__ __forceinline__ void edge_setup(const float4 v[3], int cull,
                                            int64_t r[9]) {
-  const float lim = 32768.0f;
-  const bool small =
-      fabsf(v[0].x) < lim && fabsf(v[1].x) < lim && fabsf(v[2].x) < lim &&
-      fabsf(v[0].y) < lim && fabsf(v[1].y) < lim && fabsf(v[2].y) < lim &&
-      fabsf(v[0].w) < lim && fabsf(v[1].w) < lim && fabsf(v[2].w) < lim;
   int64_t m[9];
-  if (small) {
-    const float c[9] = {v[0].x, v[1].x, v[2].x, v[0].y, v[1].y,
-                        v[2].y, v[0].w, v[1].w, v[2].w};
-#pragma unroll
-    for (int i = 0; i < 9; ++i)
-      m[i] = static_cast<int32_t>(c[i] * 65536.0f);
+  if (small_coords(v)) {
+    fixed_matrix_small(v, m);
     adjugate(m, cull, r);
   } else {
     fixed_matrix(v, m);
@@ -204,21 +211,25 @@ __device__ __forceinline__ bool cannot_cover(const int64_t m[9], int cull) {
   for (int i = 0; i < 9; ++i)
     if (m[i] >= lim || m[i] <= -lim) return false;
   if (m[6] <= 0 || m[7] <= 0 || m[8] <= 0) return false;
-  const int64_t r0 = m[4] * m[8] - m[7] * m[5];
-  const int64_t r3 = m[5] * m[6] - m[8] * m[3];
-  const int64_t r6 = m[3] * m[7] - m[6] * m[4];
+  // Every entry is below 2^20 here: products of their 32-bit values.
+  auto mul = [](int64_t a, int64_t b) {
+    return int64_t(int32_t(a)) * int64_t(int32_t(b));
+  };
+  const int64_t r0 = mul(m[4], m[8]) - mul(m[7], m[5]);
+  const int64_t r3 = mul(m[5], m[6]) - mul(m[8], m[3]);
+  const int64_t r6 = mul(m[3], m[7]) - mul(m[6], m[4]);
   const int64_t det = m[0] * r0 + m[1] * r3 + m[2] * r6;
   if (cull == DR_CULL_BACK) return det <= 0;
   // kFront: culled when det > 0 by more than the float rounding can move.
   if (det <= 0) return false;
   int64_t r[9];
   r[0] = r0; r[3] = r3; r[6] = r6;
-  r[1] = m[2] * m[7] - m[8] * m[1];
-  r[2] = m[1] * m[5] - m[4] * m[2];
-  r[4] = m[0] * m[8] - m[6] * m[2];
-  r[5] = m[2] * m[3] - m[5] * m[0];
-  r[7] = m[1] * m[6] - m[7] * m[0];
-  r[8] = m[0] * m[4] - m[3] * m[1];
+  r[1] = mul(m[2], m[7]) - mul(m[8], m[1]);
+  r[2] = mul(m[1], m[5]) - mul(m[4], m[2]);
+  r[4] = mul(m[0], m[8]) - mul(m[6], m[2]);
+  r[5] = mul(m[2], m[3]) - mul(m[5], m[0]);
+  r[7] = mul(m[1], m[6]) - mul(m[7], m[0]);
+  r[8] = mul(m[0], m[4]) - mul(m[3], m[1]);
   double bound = 0.0;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -383,7 +394,7 @@ __device__ __forceinline__ void tile_range(const Jobs& J, int64_t flat,
   if (load_triangle(job, t, v) &&
       triangle_bbox(v, job.width, job.height, b)) {
     int64_t m[9];
-    fixed_matrix(v, m);
+    if (small_coords(v)) fixed_matrix_small(v, m); else fixed_matrix(v, m);
     if (!cannot_cover(m, cull)) {
       const int tx0 = b.left / TW, ty0 = b.bottom / TH;
       nx = (b.right - 1) / TW - tx0 + 1;
