@@ -63,6 +63,27 @@ def test_interpolate_forward_and_backward(channels, device):
   assert helpers.rel_err(a.grad.cpu(), a64.grad) <= GRAD_TOL
 
 
+@pytest.mark.parametrize("channels", [2, 8, 16, 64])
+def test_interpolate_backward_ragged_frame(channels, device):
+  """The backward flushes C lanes per vertex record when C divides 64; a
+  37x23x3 frame leaves the last wave partly past the end."""
+  s, clip, ids, z, b = scene(2, 37, 23, 3, cull=0)
+  rng = np.random.default_rng(100 + channels)
+  attr = rng.normal(size=(clip.shape[0], channels)).astype(np.float32)
+  fbuf = rasterize.rasterize_triangles(
+      torch.as_tensor(clip, device=device),
+      torch.as_tensor(s["triangles"], device=device), None, 37, 23, 3, 0)
+  a = torch.as_tensor(attr, device=device).requires_grad_(True)
+  out = interpolate.interpolate_vertex_attribute(a, fbuf, None)
+  g = rng.normal(size=out.shape).astype(np.float32)
+  out.backward(torch.as_tensor(g, device=device))
+  a64 = torch.as_tensor(attr, dtype=torch.float64).requires_grad_(True)
+  w64 = so.interpolate(a64, ids, b, s["triangles"], None)
+  w64.backward(torch.as_tensor(g, dtype=torch.float64))
+  assert a64.grad.abs().sum() > 0
+  assert helpers.rel_err(a.grad.cpu(), a64.grad) <= GRAD_TOL
+
+
 def _splat_inputs(seed, w, h, layers, channels):
   s, clip, ids, z, b = scene(seed, w, h, layers)
   cbuf = so.interpolate(torch.as_tensor(clip), ids, b, s["triangles"],
