@@ -625,7 +625,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
       ss[e] = sv;
       if (S::kStageG) {
 #pragma unroll
-        for (int i = 0; i < GS / 4; ++i) sg[e * (GS / 4) + i] = g4[i];
+        for (int i = 0; i < GS / 4; ++i) sg[i * kPatch + e] = g4[i];
       }
     }
   }
@@ -677,10 +677,12 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
             layer_class(int(int8_t(surf[t >> 2] >> (8 * (t & 3)))), l);
         float gv[GS];
         if (S::kStageG && l == 0 && cls == 1) {
-          const float4* gs = stage_g + patch_index(t) * (GS / 4);
+          // One float4 plane per quarter of the class record: lanes
+          // reading neighbouring patch entries meet distinct LDS banks.
+          const float4* gs = stage_g + patch_index(t);
 #pragma unroll
           for (int c = 0; c < GS; c += 4) {
-            const float4 q4 = gs[c / 4];
+            const float4 q4 = gs[(c / 4) * kPatch];
             gv[c] = q4.x; gv[c + 1] = q4.y; gv[c + 2] = q4.z;
             gv[c + 3] = q4.w;
           }
