@@ -519,14 +519,26 @@ constexpr int kHashSlots = 256;   // > 3 x 64 entries per wave and layer
 // about a quarter as many; further distinct vertices are flushed directly).
 constexpr int kRows = 96;
 
+// A fragment's record for its corner vertex v is b * base: base holds
+// (d position or d clip [4], d colour [C]); columns past 4 + C are zero.
+// Formed where it is used, so the three corners' records are never all
+// live in registers.
 template <int C>
-__device__ __forceinline__ void flush_record(float* grads, int v,
-                                             const float* rec) {
+__device__ __forceinline__ float record_entry(float b, const float* base,
+                                              int c) {
+  return c < 4 + C ? b * base[c] : 0.0f;
+}
+
+template <int C>
+__device__ __forceinline__ void flush_record(float* grads, int v, float b,
+                                             const float* base) {
   constexpr int R = Record<C>::kSize;
   if (grads == nullptr) return;
 #pragma unroll
-  for (int c = 0; c < R; ++c)
-    if (rec[c] != 0.0f) gatomic_add(grads + int64_t(v) * R + c, rec[c]);
+  for (int c = 0; c < R; ++c) {
+    const float e = record_entry<C>(b, base, c);
+    if (e != 0.0f) gatomic_add(grads + int64_t(v) * R + c, e);
+  }
 }
 
 // The wave's 16x4 sources splat onto an 18x6 patch of targets.  Their
@@ -652,7 +664,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   for (int l = 0; l < L; ++l) {
     Corner k = k0;
     const bool frag = l == 0 ? frag0 : any && corners(J, l * P + p, k);
-    float rec[3][R];
+    float base[4 + C];
     if (frag) {
       float sx, sy, cx, cy, cw;
       splat_position(J, k, sx, sy, cx, cy, cw);
@@ -725,14 +737,9 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         dp[0] = dcx; dp[1] = dcy; dp[2] = 0.0f; dp[3] = dcw;
       }
 #pragma unroll
-      for (int i = 0; i < 3; ++i) {
+      for (int c = 0; c < 4; ++c) base[c] = dp[c];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) rec[i][c] = k.b[i] * dp[c];
-#pragma unroll
-        for (int c = 0; c < C; ++c) rec[i][4 + c] = k.b[i] * dcol[c];
-#pragma unroll
-        for (int c = 4 + C; c < R; ++c) rec[i][c] = 0.0f;
-      }
+      for (int c = 0; c < C; ++c) base[4 + c] = dcol[c];
     }
     // Wave-local dedupe into compact LDS rows (one per distinct vertex, in
     // the order first occurrences claim them), then a grouped flush.
@@ -782,9 +789,10 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         if (r < kRows) {
           vtx[r] = k.v[i];
 #pragma unroll
-          for (int c = 0; c < R; ++c) val[r * RS + c] = rec[i][c];
+          for (int c = 0; c < R; ++c)
+            val[r * RS + c] = record_entry<C>(k.b[i], base, c);
         } else {
-          flush_record<C>(grads, k.v[i], rec[i]);
+          flush_record<C>(grads, k.v[i], k.b[i], base);
         }
       }
       nr += __popcll(fm);
@@ -797,12 +805,14 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         if (first[i]) continue;
         const int r = hrow[slot[i]];
         if (r < 0) {
-          flush_record<C>(grads, k.v[i], rec[i]);
+          flush_record<C>(grads, k.v[i], k.b[i], base);
           continue;
         }
 #pragma unroll
-        for (int c = 0; c < R; ++c)
-          if (rec[i][c] != 0.0f) atomicAdd(&val[r * RS + c], rec[i][c]);
+        for (int c = 0; c < R; ++c) {
+          const float e = record_entry<C>(k.b[i], base, c);
+          if (e != 0.0f) atomicAdd(&val[r * RS + c], e);
+        }
       }
     }
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
