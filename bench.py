@@ -296,6 +296,20 @@ def grad_check(w, args, dev):
     a = a.cpu().double()
     return float((a - b).abs().max() / b.abs().max())
 
+  # Image of view 0 against the float64 oracle (before the difference
+  # quotients below re-render it with moved vertices): the fused path, and
+  # the plain per-view path, which forms the splat offsets from float32
+  # viewport coordinates as the reference does (splat.py:218-221, 103-104).
+  from diffren_amd import render
+  out64 = out.detach()
+  image_err = rel(r.images[0], out64)
+  with torch.no_grad():
+    plain = render.render_triangles(
+        w["mesh"]["vertices"], {"c": w["mesh"]["colors"]},
+        w["mesh"]["triangles"], w["cams_t"][0], R, R, lambda a: a["c"],
+        num_layers=L)
+  plain_image_err = rel(plain, out64)
+
   # eps = 1e-3 world units moves a fragment by up to 0.3-0.44 px here, about
   # 0.6-0.9 sigma of the splat's Gaussian (sigma 0.5 px, splat.py:65-66), so
   # the central difference carries a large truncation error (it falls as
@@ -308,6 +322,8 @@ def grad_check(w, args, dev):
               vertex_grad_fd_rel_err=fd,
               color_grad_rel_err=rel(r.d_colors[0], c64.grad),
               loss_rel_err=abs(r.loss.item() - loss.item()) / abs(loss.item()),
+              image_rel_err=image_err,
+              plain_image_rel_err=plain_image_err,
               triangle_ids_bit_exact=ids_exact)
 
 

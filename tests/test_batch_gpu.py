@@ -221,6 +221,38 @@ def test_uneven_batch_autotuned_matches_per_view(device):
   assert not overflow and need > 0
 
 
+def test_large_frames_match_per_view(device):
+  """Frames large enough that the raster, target and source kernels take
+  their tiles in XCD-ordered runs over many full groups plus a ragged tail
+  (1000x700 and 648x520: 2772 + 1353 raster tiles of 16x16, 2800 + 1430
+  splat tiles of 64x4), vs per-view renders of the plain path."""
+  s = scenes.icosphere_scene(seed=4, subdivisions=4)
+  mesh = dict(vertices=torch.as_tensor(s["vertices"], device=device),
+              triangles=torch.as_tensor(s["triangles"], device=device),
+              colors=torch.as_tensor(s["colors"], device=device))
+  cam = torch.as_tensor(s["camera"], device=device)
+  jobs = [(0, cam, 1000, 700), (0, cam, 648, 520)]
+  r = batch.BatchedSplatRenderer([mesh], jobs, 3, tile=(16, 16), streams=2)
+  for t in r.targets:
+    t.uniform_()
+  r.step()
+  torch.cuda.synchronize()
+  clip = so.transform(s["camera"], s["vertices"], torch.float32).numpy()
+  for (mi, c, w, h), ids in zip(jobs, r.ids):
+    want, _, _ = raster_oracle.rasterize(clip, s["triangles"], w, h, 3, 1)
+    assert np.array_equal(ids.cpu().numpy(), want)
+  imgs, loss, dv, dc = _unfused([mesh], jobs, 3, r.targets)
+  # At x ~ 1000 px a float32 viewport coordinate is rounded by up to 6e-5 px;
+  # the plain path forms floor(x) + 0.5 - x from it, the fused path from the
+  # clip values directly, so the two images differ by a few 1e-5 here (a
+  # missed or doubled tile would differ by O(1)).
+  for a, b in zip(r.images, imgs):
+    assert helpers.rel_err(a.cpu(), b.detach().cpu()) <= 1e-4
+  assert abs(r.loss.item() - loss.item()) <= 1e-5 * abs(loss.item())
+  for a, b in zip(r.d_vertices + r.d_colors, dv + dc):
+    assert helpers.rel_err(a.cpu(), b.cpu()) <= 1e-4
+
+
 def test_vertex_gradient_matches_central_differences(device):
   # Central differences of the fused path's own loss (eps 1e-3, rasterization
   # frozen) for 64 seeded vertices x 3 coordinates.
