@@ -317,7 +317,7 @@ def grad_check(w, args, dev):
   # step (the frozen barycentrics put it at the pixel centre).
   fd = {str(eps): finite_differences(r, w["mesh"]["vertices"],
                                      w["targets"][0], 1.0 / (R * R * 4),
-                                     eps=eps) for eps in (1e-3, 1e-4)}
+                                     eps=eps) for eps in (1e-3, 3e-4, 1e-4)}
   return dict(vertex_grad_rel_err=rel(r.d_vertices[0], v64.grad),
               vertex_grad_fd_rel_err=fd,
               color_grad_rel_err=rel(r.d_colors[0], c64.grad),
