@@ -477,6 +477,8 @@ __global__ __launch_bounds__(kBlock) void bin_kernel(
 
 // Exclusive scan of counts[0..n) -> offsets, in two kernels: per-block scans
 // with block totals, then each block adds the sum of the totals before it.
+// The local scan also zeroes the counts it has read: the fill pass reuses
+// them as per-tile cursors (no separate clearing launch).
 constexpr int kScanItems = 4;  // per thread
 constexpr int kScanTile = kBlock * kScanItems;
 
@@ -505,7 +507,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x,
 }
 
 __global__ __launch_bounds__(kBlock) void scan_local_kernel(
-    const uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets,
+    uint32_t* __restrict__ counts, uint32_t* __restrict__ offsets,
     uint32_t* __restrict__ block_sums, int64_t n) {
   const int64_t base = int64_t(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
   uint32_t v[kScanItems], s = 0;
@@ -518,7 +520,10 @@ __global__ __launch_bounds__(kBlock) void scan_local_kernel(
   uint32_t run = block_exclusive_scan(s, &total);
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
-    if (base + i < n) offsets[base + i] = run;
+    if (base + i < n) {
+      offsets[base + i] = run;
+      counts[base + i] = 0u;
+    }
     run += v[i];
   }
   if (threadIdx.x == 0) block_sums[blockIdx.x] = total;
@@ -949,11 +954,8 @@ int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
   scan_add_kernel<<<unsigned(sb), kBlock, 0, s>>>(
       w.offsets, w.block_sums, int(sb), total_tiles, w.status, capacity);
   DR_CHECK_LAUNCH("raster scan add");
-  // Second pass reuses counts as cursors after copying them away: the tile
-  // kernel needs the final counts, which equal the cursors after the fill.
-  clear_kernel<<<dim3(unsigned((clear_n + kBlock - 1) / kBlock)), kBlock, 0,
-                 s>>>(w.counts, clear_n);
-  DR_CHECK_LAUNCH("raster clear cursors");
+  // The fill pass reuses counts (zeroed by the local scan) as cursors: the
+  // tile kernel needs the final counts, which equal the cursors after it.
   if (tri_blocks > 0) {
     bin_kernel<TW, TH, true><<<tri_blocks, kBlock, 0, s>>>(
         J, total_tris, cull, w.counts, w.offsets, w.bins, capacity, w.desc);
