@@ -82,12 +82,20 @@ def build_workload(args, dev):
 
 
 def time_launches(r, stage, reps):
-  """Average duration of one launch of `stage` ("splat_source" or
-  "splat_target"), each group's launch bracketed by its own pair of HIP
-  events on the stream it is launched on (the current stream)."""
+  """Average duration (ms) of one launch of a step's stage, each group's
+  launch bracketed by its own pair of HIP events on the stream it is
+  launched on (the current stream):
+    raster_bins   binning half of the rasterizer (clear, count, 2 scans,
+                  fill) -- re-bins the current vertices;
+    raster_tiles  the tile pass alone (raster_tile_kernel) over those bins;
+    splat_target  fused target pass (target_kernel + the loss reduce);
+    splat_source  fused source pass (source_kernel)."""
   import torch
-  fn = {"splat_source": r._splat_backward,
-        "splat_target": lambda g: r._splat_forward(g, 1)}[stage]
+  from diffren_amd import _native
+  fn = {"raster_bins": lambda g: r._rasterize(g, _native.RASTER_BINS_ONLY),
+        "raster_tiles": lambda g: r._rasterize(g, _native.RASTER_TILES_ONLY),
+        "splat_target": lambda g: r._splat_forward(g, 1),
+        "splat_source": r._splat_backward}[stage]
   pairs = []
   for _ in range(reps):
     for g in r.groups:
@@ -99,6 +107,25 @@ def time_launches(r, stage, reps):
       pairs.append((a, b))
   torch.cuda.synchronize()
   return sum(a.elapsed_time(b) for a, b in pairs) / len(pairs)  # ms
+
+
+def copy_ceiling(dev, mib=256, reps=10):
+  """Measured device-to-device copy rate (GB/s, read + write bytes) of a
+  `mib` MiB buffer: the attainable HBM rate next to the datasheet peak."""
+  import torch
+  a = torch.ones(mib * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
+  b = torch.empty_like(a)
+  b.copy_(a)
+  torch.cuda.synchronize()
+  start = torch.cuda.Event(enable_timing=True)
+  end = torch.cuda.Event(enable_timing=True)
+  start.record()
+  for _ in range(reps):
+    b.copy_(a)
+  end.record()
+  torch.cuda.synchronize()
+  ms = start.elapsed_time(end) / reps
+  return 2 * a.numel() * 4 / (ms * 1e-3) / 1e9
 
 
 def time_stage(fn, reps):
@@ -296,19 +323,26 @@ def grad_check(w, args, dev):
     a = a.cpu().double()
     return float((a - b).abs().max() / b.abs().max())
 
-  # Image of view 0 against the float64 oracle (before the difference
-  # quotients below re-render it with moved vertices): the fused path, and
-  # the plain per-view path, which forms the splat offsets from float32
-  # viewport coordinates as the reference does (splat.py:218-221, 103-104).
+  # Image of view 0 (before the difference quotients below re-render it with
+  # moved vertices) against the float32 oracle restatement, which forms the
+  # viewport coordinates and splat offsets in the reference's float32 order
+  # (splat.py:103-104, 218-221): the fused path and the plain per-view path.
+  # Against float64 both carry the float32 rounding of x ~ 1000 px.
   from diffren_amd import render
   out64 = out.detach()
-  image_err = rel(r.images[0], out64)
   with torch.no_grad():
+    out32 = so.render_splat(torch.as_tensor(w["v"]),
+                            torch.as_tensor(w["colors"]), w["cams"][0], ids_o,
+                            b_o, w["t"])
     plain = render.render_triangles(
         w["mesh"]["vertices"], {"c": w["mesh"]["colors"]},
         w["mesh"]["triangles"], w["cams_t"][0], R, R, lambda a: a["c"],
         num_layers=L)
-  plain_image_err = rel(plain, out64)
+  image_err = rel(r.images[0], out32.double())
+  plain_image_err = rel(plain, out32.double())
+  image_err64 = rel(r.images[0], out64)
+  plain_image_err64 = rel(plain, out64)
+  fused_equals_plain = bool(torch.equal(r.images[0], plain))
 
   # eps = 1e-3 world units moves a fragment by up to 0.3-0.44 px here, about
   # 0.6-0.9 sigma of the splat's Gaussian (sigma 0.5 px, splat.py:65-66), so
@@ -324,6 +358,9 @@ def grad_check(w, args, dev):
               loss_rel_err=abs(r.loss.item() - loss.item()) / abs(loss.item()),
               image_rel_err=image_err,
               plain_image_rel_err=plain_image_err,
+              image_rel_err_vs_float64=image_err64,
+              plain_image_rel_err_vs_float64=plain_image_err64,
+              fused_image_equals_plain=fused_equals_plain,
               triangle_ids_bit_exact=ids_exact)
 
 
@@ -396,43 +433,60 @@ def main():
       splat_source=time_stage(r.splat_backward, reps))
   log("stage ms (all views): " + json.dumps(stage_ms))
 
-  # Algorithmic bytes per launch (all views), fp32/i32, each API-level input
-  # and output counted once.
+  # Algorithmic bytes per view of each stage, fp32/i32, API-level inputs
+  # and outputs counted once, no intermediate scratch:
+  #   rasterize  T*12 + V*16 read (triangles, clip vertices), L*P*20
+  #              written (ids, z, barycentrics);
+  #   target     (interpolate -> splat -> over -> L2 gradient) L*P*20 (ids,
+  #              z, barycentrics) + T*12 + V*16 + V*4C (clip vertices,
+  #              colours) + P*4C (target image) read, P*4C written (image);
+  #   source     (the VJP) L*P*16 (ids, barycentrics) + T*12 + V*16 + V*4C
+  #              + P*4C (d image) read, V*4*(3 + C) written (d vertices,
+  #              d colours; once per mesh per launch).
+  # The gradient scratch between the two splat passes is an intermediate and
+  # is not counted.
   P = R * R
   lp = L * P
-  # Per view: the API-level inputs and outputs of each pass, once each:
-  #   target: ids, barycentrics, z (L*P*20), triangles, clip vertices,
-  #           colours, target image -> image, gradient scratch
-  #           (3 classes x (C + 1) floats + 9 surface-index bytes per pixel);
-  #   source: ids, barycentrics (L*P*16), gradient scratch, triangles, clip
-  #           vertices, colours, camera -> vertex gradients (3 + C floats
-  #           per vertex, written once per step).
-  scratch = 3 * (C + 1) * 4 + 9
-  bytes_target = B * (lp * 20 + T * 12 + V * 16 + V * 4 * C + P * 4 * C +
-                      P * 4 * C + P * scratch)
-  bytes_source = B * (lp * 16 + P * scratch + T * 12 + V * 16 + V * 4 * C +
-                      64) + V * 4 * (C + 3)
-  bytes_raster = B * (T * 12 + V * 16 + lp * (4 + 4 + 12))
-  # The dominant single kernel (the raster stage is several kernels and is
-  # reported in stage_ms).  Stages launch once per job group.
   ng = len(r.groups)
-  per_stage = {
-      "splat_source": ("source_kernel<4> (splat / interpolate / transform "
-                       "VJP, gradient scatter)", bytes_source),
-      "splat_target": ("target_kernel<4> (interpolate + splat + over + L2 "
-                       "gradient)", bytes_target),
+  vpl = B / ng  # views per launch
+  per_launch = {
+      "raster_tiles": vpl * (T * 12 + V * 16 + lp * 20),
+      "splat_target": vpl * (lp * 20 + T * 12 + V * 16 + V * 4 * C +
+                             P * 4 * C + P * 4 * C),
+      "splat_source": vpl * (lp * 16 + T * 12 + V * 16 + V * 4 * C +
+                             P * 4 * C) + V * 4 * (3 + C),
   }
-  dom = max(per_stage, key=lambda k: stage_ms[k])
-  name, nbytes = per_stage[dom]
-  launch_ms = time_launches(r, dom, reps)
-  launch_bytes = nbytes / ng
-  achieved = launch_bytes / (launch_ms * 1e-3) / 1e9
-  roofline = dict(bound="hbm", kernel=name, achieved=round(achieved, 1),
-                  peak=HBM_PEAK_GBS, unit="GB/s",
-                  frac=round(achieved / HBM_PEAK_GBS, 4),
-                  traffic=measured_traffic(dom, ng),
-                  algorithmic_bytes_per_launch=int(launch_bytes),
-                  avg_launch_ms=round(launch_ms, 4), launches_per_step=ng)
+  kernel_names = {
+      "raster_tiles": "raster_tile_kernel<%d, %d>" % (
+          r.raster_config.tile_w, r.raster_config.tile_h),
+      "splat_target": "target_kernel<4>",
+      "splat_source": "source_kernel<4>",
+  }
+  launch_ms = {k: time_launches(r, k, reps)
+               for k in ("raster_bins", "raster_tiles", "splat_target",
+                         "splat_source")}
+  ceiling = copy_ceiling(dev)
+  kernels = {}
+  for k, nbytes in per_launch.items():
+    gbs = nbytes / (launch_ms[k] * 1e-3) / 1e9
+    kernels[k] = dict(kernel=kernel_names[k],
+                      avg_launch_ms=round(launch_ms[k], 4),
+                      algorithmic_bytes_per_launch=int(nbytes),
+                      achieved_gbs=round(gbs, 1),
+                      frac=round(gbs / HBM_PEAK_GBS, 4),
+                      frac_of_copy_ceiling=round(gbs / ceiling, 4),
+                      traffic=measured_traffic(k, ng))
+  log("launch ms: " + json.dumps(launch_ms))
+  dom = max(per_launch, key=lambda k: launch_ms[k])
+  d = kernels[dom]
+  roofline = dict(bound="hbm", kernel=d["kernel"], achieved=d["achieved_gbs"],
+                  peak=HBM_PEAK_GBS, unit="GB/s", frac=d["frac"],
+                  traffic=d["traffic"],
+                  algorithmic_bytes_per_launch=d["algorithmic_bytes_per_launch"],
+                  avg_launch_ms=d["avg_launch_ms"], launches_per_step=ng,
+                  copy_ceiling_gbs=round(ceiling, 1),
+                  binning_half_ms=round(launch_ms["raster_bins"], 4),
+                  kernels=kernels)
 
   # Plain path: one render_triangles per view + autograd (one warm-up step).
   plain_step(w, args)

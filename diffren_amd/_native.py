@@ -19,6 +19,8 @@ _c_size = ctypes.c_size_t
 
 
 RASTER_SPARSE = 1
+RASTER_BINS_ONLY = 2
+RASTER_TILES_ONLY = 4
 MLP_ALBEDO_MODULATE = 1
 PAD_CONSTANT = 0
 PAD_EDGE = 1

@@ -264,11 +264,17 @@ class BatchedSplatRenderer:
               v.shape[1], v.shape[0], _native.ptr(self.clip_blocks[mi]), s),
           "dr_transform_homogeneous_multi")
 
-  def _rasterize(self, g):
+  def _rasterize(self, g, half=0):
+    """half: 0 for the whole rasterizer, or _native.RASTER_BINS_ONLY /
+    RASTER_TILES_ONLY for one half of it."""
+    cfg = self.raster_config
+    if half:
+      cfg = _native.RasterConfig(cfg.tile_w, cfg.tile_h, cfg.bin_capacity,
+                                 cfg.flags | half, 0)
     _native.check(
         _native.lib().dr_rasterize_triangles_batched(
             _native.ptr(g.raster_table), len(g.indices), g.total_triangles,
-            g.raster_tiles, self.L, 1, ctypes.byref(self.raster_config),
+            g.raster_tiles, self.L, 1, ctypes.byref(cfg),
             _native.ptr(g.raster_ws), g.raster_ws.numel(),
             _native.stream_handle()), "dr_rasterize_triangles_batched")
 

@@ -937,30 +937,37 @@ template <int TW, int TH>
 int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
                int cull, const Workspace& w, uint64_t capacity, int flags,
                hipStream_t s) {
-  const int64_t clear_n = total_tiles;
-  clear_kernel<<<dim3(unsigned((clear_n + kBlock - 1) / kBlock)), kBlock, 0,
-                 s>>>(w.counts, clear_n);
-  DR_CHECK_LAUNCH("raster clear");
-  const unsigned tri_blocks = unsigned((total_tris + kBlock - 1) / kBlock);
-  if (tri_blocks > 0) {
-    bin_kernel<TW, TH, false><<<tri_blocks, kBlock, 0, s>>>(
-        J, total_tris, cull, w.counts, w.offsets, w.bins, capacity, w.desc);
-    DR_CHECK_LAUNCH("raster bin count");
+  // The two halves can run as separate calls on the same workspace (each
+  // call re-reads the job table): binning, then the tile pass.
+  const bool run_bins = (flags & DR_RASTER_TILES_ONLY) == 0;
+  const bool run_tiles = (flags & DR_RASTER_BINS_ONLY) == 0;
+  if (run_bins) {
+    const int64_t clear_n = total_tiles;
+    clear_kernel<<<dim3(unsigned((clear_n + kBlock - 1) / kBlock)), kBlock, 0,
+                   s>>>(w.counts, clear_n);
+    DR_CHECK_LAUNCH("raster clear");
+    const unsigned tri_blocks = unsigned((total_tris + kBlock - 1) / kBlock);
+    if (tri_blocks > 0) {
+      bin_kernel<TW, TH, false><<<tri_blocks, kBlock, 0, s>>>(
+          J, total_tris, cull, w.counts, w.offsets, w.bins, capacity, w.desc);
+      DR_CHECK_LAUNCH("raster bin count");
+    }
+    const int64_t sb = scan_blocks(total_tiles);
+    scan_local_kernel<<<unsigned(sb), kBlock, 0, s>>>(w.counts, w.offsets,
+                                                     w.block_sums, total_tiles);
+    DR_CHECK_LAUNCH("raster scan");
+    scan_add_kernel<<<unsigned(sb), kBlock, 0, s>>>(
+        w.offsets, w.block_sums, int(sb), total_tiles, w.status, capacity);
+    DR_CHECK_LAUNCH("raster scan add");
+    // The fill pass reuses counts (zeroed by the local scan) as cursors: the
+    // tile kernel needs the final counts, which equal the cursors after it.
+    if (tri_blocks > 0) {
+      bin_kernel<TW, TH, true><<<tri_blocks, kBlock, 0, s>>>(
+          J, total_tris, cull, w.counts, w.offsets, w.bins, capacity, w.desc);
+      DR_CHECK_LAUNCH("raster bin fill");
+    }
   }
-  const int64_t sb = scan_blocks(total_tiles);
-  scan_local_kernel<<<unsigned(sb), kBlock, 0, s>>>(w.counts, w.offsets,
-                                                   w.block_sums, total_tiles);
-  DR_CHECK_LAUNCH("raster scan");
-  scan_add_kernel<<<unsigned(sb), kBlock, 0, s>>>(
-      w.offsets, w.block_sums, int(sb), total_tiles, w.status, capacity);
-  DR_CHECK_LAUNCH("raster scan add");
-  // The fill pass reuses counts (zeroed by the local scan) as cursors: the
-  // tile kernel needs the final counts, which equal the cursors after it.
-  if (tri_blocks > 0) {
-    bin_kernel<TW, TH, true><<<tri_blocks, kBlock, 0, s>>>(
-        J, total_tris, cull, w.counts, w.offsets, w.bins, capacity, w.desc);
-    DR_CHECK_LAUNCH("raster bin fill");
-  }
+  if (!run_tiles) return kOk;
   const size_t lds = sizeof(uint64_t) * L * TW * TH;
   raster_tile_kernel<TW, TH><<<unsigned(total_tiles), kBlock, lds, s>>>(
       J, L, cull, w.counts, w.offsets, w.bins, capacity,
@@ -977,6 +984,10 @@ int dispatch(const dr_raster_job* table, const dr_raster_job* single,
              hipStream_t s) {
   DR_CHECK_ARG(L >= 1 && L <= kMaxLayers, "num_layers must be in [1, 8]");
   DR_CHECK_ARG(cull >= 0 && cull <= 2, "face_culling_mode must be 0, 1 or 2");
+  DR_CHECK_ARG((c.flags & DR_RASTER_BINS_ONLY) == 0 ||
+                   (c.flags & DR_RASTER_TILES_ONLY) == 0,
+               "DR_RASTER_BINS_ONLY and DR_RASTER_TILES_ONLY exclude each "
+               "other");
   DR_CHECK_ARG(valid_tile(c.tile_w, c.tile_h), "unsupported tile size");
   DR_CHECK_ARG(total_tiles > 0 && total_tiles < (int64_t(1) << 31),
                "tile count out of range");

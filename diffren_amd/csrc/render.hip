@@ -111,14 +111,23 @@ __device__ __forceinline__ float interp3(const Corner& k, float a0, float a1,
 
 // Offset from a splat to the centre of its own pixel, floor(x) + 0.5 - x,
 // where x = ((c / w + 1) * size) * 0.5 is its viewport coordinate
-// (splat.py:214-221, 103-104).  In rasterize-then-splat x sits on the pixel
-// centre up to rounding, so this offset is pure rounding noise of ~1e-5 px at
-// x ~ 1000 when x is formed in float32 -- yet it is what the tap weights'
-// position gradients are evaluated at.  It is computed here as
+// (splat.py:214-221, 103-104).  The forward (target pass) forms it exactly
+// as the reference does, in float32 and in that order, so the image is the
+// reference's float32 image.
+__device__ __forceinline__ float reference_offset(float c, float w, int size) {
+  const float x = ((c / w + 1.0f) * float(size)) * 0.5f;
+  return (floorf(x) + 0.5f) - x;
+}
+
+// The same offset for the backward.  In rasterize-then-splat x sits on the
+// pixel centre up to rounding, so reference_offset is pure rounding noise of
+// ~6e-5 px at x ~ 1000 -- yet it is what the tap weights' position
+// gradients are evaluated at.  The source pass computes it as
 //   -(a c + k w) / w,  a = size / 2, k = a - floor(x) - 0.5 (both exact),
 // with the product k w split exactly by an fma (two-product), so the
 // cancellation happens without rounding and the offset is accurate to a few
-// ulps of itself.  The forward image is unchanged to far below 1e-5.
+// ulps of itself: the gradient is then the derivative at the exact splat
+// position, as the float64 oracle evaluates it.
 __device__ __forceinline__ float center_offset(float c, float w, int size) {
   const float a = 0.5f * float(size);
   const float x = ((c / w + 1.0f) * float(size)) * 0.5f;
@@ -147,6 +156,9 @@ __device__ __forceinline__ float4 clip_vertex(const dr_splat_job& J, int v) {
   return r;
 }
 
+// kForward: the reference's float32 offsets (target pass); otherwise the
+// compensated ones (source pass).
+template <bool kForward>
 __device__ __forceinline__ void splat_position(const dr_splat_job& J,
                                                const Corner& k, float& ox,
                                                float& oy, float& cx,
@@ -156,8 +168,13 @@ __device__ __forceinline__ void splat_position(const dr_splat_job& J,
   cx = interp3(k, p0.x, p1.x, p2.x);
   cy = interp3(k, p0.y, p1.y, p2.y);
   cw = interp3(k, p0.w, p1.w, p2.w);
-  ox = center_offset(cx, cw, J.width);
-  oy = center_offset(cy, cw, J.height);
+  if (kForward) {
+    ox = reference_offset(cx, cw, J.width);
+    oy = reference_offset(cy, cw, J.height);
+  } else {
+    ox = center_offset(cx, cw, J.width);
+    oy = center_offset(cy, cw, J.height);
+  }
 }
 
 __device__ __forceinline__ float viewport_z(float z) {
@@ -333,7 +350,7 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
         good = good && unsigned(k[j].v[e]) < unsigned(J.num_vertices);
       if (good) {
         float cx, cy, cw;
-        splat_position(J, k[j], px, py, cx, cy, cw);
+        splat_position<true>(J, k[j], px, py, cx, cy, cw);
         const float* a0 = J.colors + int64_t(k[j].v[0]) * C;
         const float* a1 = J.colors + int64_t(k[j].v[1]) * C;
         const float* a2 = J.colors + int64_t(k[j].v[2]) * C;
@@ -667,7 +684,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     float base[4 + C];
     if (frag) {
       float sx, sy, cx, cy, cw;
-      splat_position(J, k, sx, sy, cx, cy, cw);
+      splat_position<false>(J, k, sx, sy, cx, cy, cw);
       const float* a0 = J.colors + int64_t(k.v[0]) * C;
       const float* a1 = J.colors + int64_t(k.v[1]) * C;
       const float* a2 = J.colors + int64_t(k.v[2]) * C;

@@ -96,6 +96,13 @@ typedef struct dr_raster_config {
  * render), it saves the background writes of every layer. */
 enum { DR_RASTER_SPARSE = 1 };
 
+/* A call normally bins the triangles into screen tiles and then rasterizes
+ * the tiles.  DR_RASTER_BINS_ONLY runs the binning half only; a later call
+ * with DR_RASTER_TILES_ONLY and the same jobs, workspace and config runs the
+ * tile half over those bins (the vertices must not change in between).  For
+ * timing or overlapping the halves separately. */
+enum { DR_RASTER_BINS_ONLY = 2, DR_RASTER_TILES_ONLY = 4 };
+
 /* Host: fills triangle_begin / tile_begin of jobs_host[0..num_jobs) and
  * returns the totals.  Returns DR_OK or DR_INVALID_ARGUMENT. */
 int dr_raster_jobs_prepare(dr_raster_job* jobs_host, int32_t num_jobs,

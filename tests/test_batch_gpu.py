@@ -242,12 +242,11 @@ def test_large_frames_match_per_view(device):
     want, _, _ = raster_oracle.rasterize(clip, s["triangles"], w, h, 3, 1)
     assert np.array_equal(ids.cpu().numpy(), want)
   imgs, loss, dv, dc = _unfused([mesh], jobs, 3, r.targets)
-  # At x ~ 1000 px a float32 viewport coordinate is rounded by up to 6e-5 px;
-  # the plain path forms floor(x) + 0.5 - x from it, the fused path from the
-  # clip values directly, so the two images differ by a few 1e-5 here (a
-  # missed or doubled tile would differ by O(1)).
+  # Both paths form the splat offsets floor(x) + 0.5 - x from float32
+  # viewport coordinates in the reference's order (splat.py:103-104,
+  # 218-221) and splat in the same order: the images are identical.
   for a, b in zip(r.images, imgs):
-    assert helpers.rel_err(a.cpu(), b.detach().cpu()) <= 1e-4
+    assert torch.equal(a, b.detach())
   assert abs(r.loss.item() - loss.item()) <= 1e-5 * abs(loss.item())
   for a, b in zip(r.d_vertices + r.d_colors, dv + dc):
     assert helpers.rel_err(a.cpu(), b.cpu()) <= 1e-4
