@@ -248,8 +248,23 @@ def test_large_frames_match_per_view(device):
   for a, b in zip(r.images, imgs):
     assert torch.equal(a, b.detach())
   assert abs(r.loss.item() - loss.item()) <= 1e-5 * abs(loss.item())
-  for a, b in zip(r.d_vertices + r.d_colors, dv + dc):
-    assert helpers.rel_err(a.cpu(), b.cpu()) <= 1e-4
+  # Gradients against the float64 oracle.  The fused backward evaluates the
+  # tap derivatives at the exact splat offset (compensated product); the
+  # plain backward, like the reference's float32 autodiff, at the offset
+  # formed from the rounded float32 viewport x (up to 6e-5 px off at
+  # x ~ 1000), so it is held to 3e-4 here.
+  v64 = torch.as_tensor(s["vertices"], dtype=torch.float64).requires_grad_()
+  c64 = torch.as_tensor(s["colors"], dtype=torch.float64).requires_grad_()
+  total = 0.0
+  for (mi, c, w, h), tgt in zip(jobs, r.targets):
+    ids, _, b = raster_oracle.rasterize(clip, s["triangles"], w, h, 3, 1)
+    o = so.render_splat(v64, c64, s["camera"], ids, b, s["triangles"])
+    total = total + ((o - tgt.cpu().double()) ** 2).sum() * r.loss_scale
+  total.backward()
+  for a, p, want in ((r.d_vertices[0], dv[0], v64.grad),
+                     (r.d_colors[0], dc[0], c64.grad)):
+    assert helpers.rel_err(a.cpu(), want) <= 1e-4
+    assert helpers.rel_err(p.cpu(), want) <= 3e-4
 
 
 def test_vertex_gradient_matches_central_differences(device):
