@@ -74,6 +74,11 @@ SIGNATURES = {
                                         _c_int, _c_int, _c_int, _c_int,
                                         _c_ptr, _c_ptr, _c_ptr, _c_ptr,
                                         _c_ptr, _c_size, _c_ptr]),
+    "dr_rasterize_triangles_noalloc": (_c_int, [_c_ptr, _c_int, _c_ptr,
+                                                _c_int, _c_int, _c_int,
+                                                _c_int, _c_int, _c_ptr,
+                                                _c_ptr, _c_ptr, _c_ptr,
+                                                _c_ptr]),
     "dr_rasterize_triangles_batched": (_c_int, [_c_ptr, _c_int, _c_i64,
                                                 _c_i64, _c_int, _c_int,
                                                 _c_ptr, _c_ptr, _c_size,

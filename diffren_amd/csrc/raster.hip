@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "dr_common.h"
 #include "../../include/diffren_amd.h"
@@ -1025,6 +1026,41 @@ int dispatch(const dr_raster_job* table, const dr_raster_job* single,
   return fail(kUnsupported, "unsupported tile size");
 }
 
+// Workspace of dr_rasterize_triangles_noalloc: a file-scope device array,
+// one copy per GPU, held until the process ends.  In host code `scratch`
+// names a host-side stand-in; the device address of each GPU's copy comes
+// from hipGetSymbolAddress, looked up once per device with that device
+// current, and only that address is handed to kernels.
+constexpr size_t kStaticScratchBytes = DR_STATIC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+}  // namespace dr
+
+static __device__ __align__(256) unsigned char scratch[DR_STATIC_SCRATCH_BYTES];
+
+namespace dr {
+namespace {
+
+// Device address of the current device's `scratch`.
+int static_scratch(void** out) {
+  int dev = -1;
+  DR_CHECK_HIP(hipGetDevice(&dev));
+  DR_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    DR_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    DR_CHECK_ARG(p != nullptr && (reinterpret_cast<uintptr_t>(p) & 255) == 0,
+                 "static scratch has no aligned device address");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
+
 dr_raster_config resolve_config(const dr_raster_config* c) {
   dr_raster_config r{32, 8, 0, 0, 0};
   if (c) r = *c;
@@ -1099,6 +1135,45 @@ int dr_rasterize_triangles(const float* vertices, int32_t num_vertices,
   if (rc != DR_OK) return rc;
   return dr::dispatch(nullptr, &job, 1, tt, tiles, num_layers,
                       face_culling_mode, c, workspace, workspace_bytes,
+                      static_cast<hipStream_t>(stream));
+}
+
+int dr_rasterize_triangles_noalloc(const float* vertices, int32_t num_vertices,
+                                   const int32_t* triangles,
+                                   int32_t num_triangles, int32_t width,
+                                   int32_t height, int32_t num_layers,
+                                   int32_t face_culling_mode,
+                                   int32_t* triangle_ids, float* z_buffer,
+                                   float* barycentrics,
+                                   const dr_raster_config* config,
+                                   dr_stream_t stream) {
+  const dr_raster_config c = dr::resolve_config(config);
+  dr_raster_job job{};
+  job.vertices = vertices;
+  job.triangles = triangles;
+  job.triangle_ids = triangle_ids;
+  job.z_buffer = z_buffer;
+  job.barycentrics = barycentrics;
+  job.num_vertices = num_vertices;
+  job.num_triangles = num_triangles;
+  job.width = width;
+  job.height = height;
+  int64_t tt = 0, tiles = 0;
+  int rc = dr_raster_jobs_prepare(&job, 1, &c, &tt, &tiles);
+  if (rc != DR_OK) return rc;
+  const size_t need = dr_raster_workspace_size(tt, tiles, &c);
+  if (need > dr::kStaticScratchBytes)
+    return dr::fail(dr::kWorkspaceTooSmall,
+                    "dr_rasterize_triangles_noalloc: this call needs " +
+                        std::to_string(need) +
+                        " bytes of scratch, above the static scratch of " +
+                        std::to_string(dr::kStaticScratchBytes) +
+                        " bytes; use dr_rasterize_triangles with a workspace");
+  void* ws = nullptr;
+  rc = dr::static_scratch(&ws);
+  if (rc != DR_OK) return rc;
+  return dr::dispatch(nullptr, &job, 1, tt, tiles, num_layers,
+                      face_culling_mode, c, ws, dr::kStaticScratchBytes,
                       static_cast<hipStream_t>(stream));
 }
 

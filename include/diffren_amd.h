@@ -130,6 +130,29 @@ int dr_rasterize_triangles(const float* vertices, int32_t num_vertices,
                            const dr_raster_config* config, void* workspace,
                            size_t workspace_bytes, dr_stream_t stream);
 
+/* Bytes of the static scratch that dr_rasterize_triangles_noalloc uses. */
+#define DR_STATIC_SCRATCH_BYTES (64u << 20)
+
+/* dr_rasterize_triangles without a workspace argument: the scratch is a
+ * file-scope device array of DR_STATIC_SCRATCH_BYTES (64 MiB) per GPU, held
+ * until the process ends, so the call allocates nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, handlers that forbid allocation).
+ * A call whose workspace (dr_raster_workspace_size) exceeds the scratch
+ * returns DR_WORKSPACE_TOO_SMALL.  The stream must belong to the current
+ * device.  Every call on a GPU shares that GPU's scratch: calls that overlap
+ * on one GPU (different streams, or host threads) race and must be
+ * serialised by the caller.  Results are those of dr_rasterize_triangles. */
+int dr_rasterize_triangles_noalloc(const float* vertices, int32_t num_vertices,
+                                   const int32_t* triangles,
+                                   int32_t num_triangles, int32_t width,
+                                   int32_t height, int32_t num_layers,
+                                   int32_t face_culling_mode,
+                                   int32_t* triangle_ids, float* z_buffer,
+                                   float* barycentrics,
+                                   const dr_raster_config* config,
+                                   dr_stream_t stream);
+
 /* Rasterizes many (mesh, camera, resolution) jobs in one set of launches.
  * jobs_device: device copy of a table prepared by dr_raster_jobs_prepare.
  * No reference counterpart: the reference batches with
