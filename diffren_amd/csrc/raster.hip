@@ -369,108 +369,255 @@ __global__ void clear_kernel(uint32_t* p, int64_t n) {
 }
 
 
-// Pass 1 (kFill == false): count entries per tile into counter[] and record
-// each triangle's tile range in desc[]:
-//   desc.x = first tile (flat over all jobs),
-//   desc.y = nx | ny << 8 | tiles-per-row << 16 when nx, ny < 256,
-//            0 when the triangle covers no tile, 0xffffffff to recompute.
-// Pass 2 (kFill == true): append triangle indices to their tiles' bins from
-// desc[] (no vertex gathers or fixed-point set-up), using counter[] as
-// per-tile cursors and offsets[] from the scan.
-// Lanes of a wave that hit the same tile share one atomic (ballot, then ranks
-// from popcounts), which matters because neighbouring triangles of a mesh
-// mostly land in the same tile.
-constexpr uint32_t kRecompute = 0xffffffffu;
+// Binning.  The count pass visits every triangle once (one lane each):
+// gathers its vertices, clips its bounding box, and rejects it when culling
+// leaves it no pixel.  A triangle whose clipped box is at most 4 x 4 pixels
+// (nearly every triangle of a dense mesh) is rasterized right there: its
+// fixed-point set-up and edge tests run once, and when it covers at most six
+// pixel centres its fragments -- pixel and z, the same arithmetic as the
+// tile pass -- are kept in a compact per-triangle record; a triangle that
+// covers none leaves no entry at all.  Larger triangles (or small ones with
+// more fragments) are binned as whole triangles into every tile their box
+// touches, and the tile pass walks their pixels.
+//
+// Per-triangle record, two planes:
+//   head[t] = (w0, w1):
+//     w0 == 0            no entry;
+//     w0 & kHeadFrag     n = w0 & 7 fragments (1..6) with 4-bit offsets
+//                        (dy << 2 | dx) from the box origin in bits 3..26,
+//                        w1 = origin x | origin y << 16;
+//     w0 & kHeadTri      nx = w0 & 255 by ny = (w0 >> 8) & 255 tiles from
+//                        first tile w1 (flat over all jobs), tiles per row
+//                        (w0 >> 16) & 8191; kHeadRecompute: recompute them.
+//   frag_z[t][8]: the fragments' z (float bits), written for fragments only.
+// Each (tile, entry) of the count pass adds one to counter[tile]; after the
+// scan the fill pass replays the same entries from the records (no vertex
+// gathers) and appends them to their tiles' bins: a 64-bit key
+// (z order << 32 | triangle) with the pixel's index in the tile, or
+// (kTriMarker << 32 | triangle) for a whole triangle.  Lanes of a wave that
+// hit the same tile share one atomic (ballot, then ranks from popcounts).
+constexpr uint32_t kHeadFrag = 0x80000000u;
+constexpr uint32_t kHeadTri = 0x40000000u;
+constexpr uint32_t kHeadRecompute = 0x20000000u;
+constexpr int kInlineFrags = 6;
+constexpr int kSmallSide = 4;
+// High word of a whole-triangle bin entry: no storable z has this order key
+// (z < 1 maps below 0xbf800000).
+constexpr uint32_t kTriMarker = 0xffffffffu;
+
+// Entries of one lane's triangle.
+struct LaneEntries {
+  uint32_t w0 = 0, w1 = 0;  // record head
+  int n = 0;                // entries
+  int t = 0;                // triangle index in its job
+  // fragments: the job's first tile and tiles per row
+  int64_t tile_begin = 0;
+  int ntx = 1;
+  // whole triangle: first tile, nx tiles per tile row, tiles per image row
+  uint32_t tile0 = 0;
+  int nx = 1;
+  uint32_t tpr = 1;
+};
 
 template <int TW, int TH>
-__device__ __forceinline__ void tile_range(const Jobs& J, int64_t flat,
-                                           int cull, int& t, int& n, int& nx,
-                                           int& ntx, uint32_t& tile0) {
-  const int j = find_job<false>(J, flat);
-  const dr_raster_job& job = J.table[j];
-  t = int(flat - job.triangle_begin);
-  n = 0;
-  float4 v[3];
-  Box b;
-  if (load_triangle(job, t, v) &&
-      triangle_bbox(v, job.width, job.height, b)) {
-    int64_t m[9];
-    if (small_coords(v)) fixed_matrix_small(v, m); else fixed_matrix(v, m);
-    if (!cannot_cover(m, cull)) {
-      const int tx0 = b.left / TW, ty0 = b.bottom / TH;
-      nx = (b.right - 1) / TW - tx0 + 1;
-      n = nx * ((b.top - 1) / TH - ty0 + 1);
-      ntx = tiles_x(job, TW);
-      tile0 = uint32_t(job.tile_begin + int64_t(ty0) * ntx + tx0);
+__device__ __forceinline__ uint32_t entry_tile(const LaneEntries& e, int k,
+                                               int& pix) {
+  if (e.w0 & kHeadFrag) {
+    const uint32_t o = (e.w0 >> (3 + 4 * k)) & 15u;
+    const int px = int(e.w1 & 0xffffu) + int(o & 3u);
+    const int py = int(e.w1 >> 16) + int(o >> 2);
+    const int tx = px / TW, ty = py / TH;
+    pix = (py - ty * TH) * TW + (px - tx * TW);
+    return uint32_t(e.tile_begin + int64_t(ty) * e.ntx + tx);
+  }
+  pix = 0;
+  return e.tile0 + uint32_t(k / e.nx) * e.tpr + uint32_t(k % e.nx);
+}
+
+// Tile range of a whole triangle with clipped box b.
+template <int TW, int TH>
+__device__ __forceinline__ void whole_triangle(const dr_raster_job& job,
+                                               const Box& b, LaneEntries& e) {
+  const int tx0 = b.left / TW, ty0 = b.bottom / TH;
+  const int ny = (b.top - 1) / TH - ty0 + 1;
+  e.nx = (b.right - 1) / TW - tx0 + 1;
+  e.n = e.nx * ny;
+  e.tpr = uint32_t(tiles_x(job, TW));
+  e.tile0 = uint32_t(job.tile_begin + int64_t(ty0) * e.tpr + tx0);
+  e.w1 = e.tile0;
+  e.w0 = (e.nx < 256 && ny < 256 && e.tpr < 8192)
+             ? kHeadTri | uint32_t(e.nx) | (uint32_t(ny) << 8) |
+                   (e.tpr << 16)
+             : kHeadTri | kHeadRecompute;
+}
+
+// Whole-triangle entries from a record head (recomputed from the vertices
+// when the range did not fit the head).
+template <int TW, int TH>
+__device__ __forceinline__ void whole_from_head(const Jobs& J, int64_t flat,
+                                                LaneEntries& e) {
+  if (e.w0 & kHeadRecompute) {
+    const dr_raster_job& job = J.table[find_job<false>(J, flat)];
+    float4 v[3];
+    Box b;
+    e.n = 0;
+    if (load_triangle(job, e.t, v) &&
+        triangle_bbox(v, job.width, job.height, b))
+      whole_triangle<TW, TH>(job, b, e);
+    return;
+  }
+  e.nx = int(e.w0 & 255u);
+  e.n = e.nx * int((e.w0 >> 8) & 255u);
+  e.tpr = (e.w0 >> 16) & 8191u;
+  e.tile0 = e.w1;
+}
+
+// Groups the lanes of a wave by tile with ballots alone (no memory
+// traffic): each lane with `has` learns the lowest lane of its group, its
+// rank in the group and the group size.
+__device__ __forceinline__ void group_by_tile(bool has, uint32_t tile,
+                                              int& leader, int& rank,
+                                              int& size) {
+  uint64_t todo = __ballot(has);
+  leader = rank = size = 0;
+  while (todo) {
+    const int first = __ffsll(static_cast<unsigned long long>(todo)) - 1;
+    const uint32_t lt = __builtin_amdgcn_readlane(tile, first);
+    const bool mine = has && tile == lt;
+    const uint64_t peers = __ballot(mine);
+    if (mine) {
+      leader = first;
+      rank = __popcll(peers & lanemask_lt());
+      size = __popcll(peers);
     }
+    todo &= ~peers;
   }
 }
 
-template <int TW, int TH, bool kFill>
-__global__ __launch_bounds__(kBlock) void bin_kernel(
+template <int TW, int TH>
+__global__ __launch_bounds__(kBlock) void bin_count_kernel(
     Jobs J, int64_t total_tris, int cull, uint32_t* __restrict__ counter,
-    const uint32_t* __restrict__ offsets, uint32_t* __restrict__ bins,
-    uint64_t capacity, uint2* __restrict__ desc) {
+    uint2* __restrict__ head, uint4* __restrict__ frag_z) {
   const int64_t flat = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-  int n = 0, nx = 1, t = 0, ntx = 1;
-  uint32_t tile0 = 0;
+  LaneEntries e;
   if (flat < total_tris) {
-    if (!kFill) {
-      tile_range<TW, TH>(J, flat, cull, t, n, nx, ntx, tile0);
-      const int ny = n > 0 ? n / nx : 0;
-      uint32_t shape = 0;
-      if (n > 0)
-        shape = (nx < 256 && ny < 256 && ntx < 65536)
-                    ? uint32_t(nx) | (uint32_t(ny) << 8) |
-                          (uint32_t(ntx) << 16)
-                    : kRecompute;
-      desc[flat] = make_uint2(tile0, shape);
-    } else {
-      const uint2 d = desc[flat];
-      if (d.y == kRecompute) {
-        tile_range<TW, TH>(J, flat, cull, t, n, nx, ntx, tile0);
-      } else if (d.y != 0) {
-        const int j = find_job<false>(J, flat);
-        t = int(flat - J.table[j].triangle_begin);
-        nx = int(d.y & 0xff);
-        n = nx * int((d.y >> 8) & 0xff);
-        ntx = int(d.y >> 16);
-        tile0 = d.x;
+    const dr_raster_job& job = J.table[find_job<false>(J, flat)];
+    e.t = int(flat - job.triangle_begin);
+    e.tile_begin = job.tile_begin;
+    e.ntx = tiles_x(job, TW);
+    float4 v[3];
+    Box b;
+    if (load_triangle(job, e.t, v) &&
+        triangle_bbox(v, job.width, job.height, b)) {
+      int64_t m[9];
+      if (small_coords(v)) fixed_matrix_small(v, m); else fixed_matrix(v, m);
+      if (!cannot_cover(m, cull)) {
+        bool whole = true;
+        if (b.right - b.left <= kSmallSide && b.top - b.bottom <= kSmallSide) {
+          // The tile pass's arithmetic exactly (edge_setup's adjugate of the
+          // same fixed-point matrix, shade_pixel).
+          int64_t r[9];
+          adjugate(m, cull, r);
+          const float sx2 = 2.0f / static_cast<float>(job.width);
+          const float sy2 = 2.0f / static_cast<float>(job.height);
+          int nf = 0;
+          uint32_t offs = 0;
+          uint32_t z[kInlineFrags] = {0u, 0u, 0u, 0u, 0u, 0u};
+          for (int iy = b.bottom; iy < b.top; ++iy)
+            for (int ix = b.left; ix < b.right; ++ix) {
+              Fragment f;
+              if (!shade_pixel(ix, iy, sx2, sy2, r, v, f)) continue;
+              if (nf < kInlineFrags) {
+                offs |= uint32_t(((iy - b.bottom) << 2) | (ix - b.left))
+                        << (4 * nf);
+#pragma unroll
+                for (int k = 0; k < kInlineFrags; ++k)
+                  if (k == nf) z[k] = __float_as_uint(f.z);
+              }
+              ++nf;
+            }
+          if (nf <= kInlineFrags) {
+            whole = false;
+            e.n = nf;
+            if (nf > 0) {
+              e.w0 = kHeadFrag | uint32_t(nf) | (offs << 3);
+              e.w1 = uint32_t(b.left) | (uint32_t(b.bottom) << 16);
+              frag_z[2 * flat] = make_uint4(z[0], z[1], z[2], z[3]);
+              if (nf > 4) frag_z[2 * flat + 1] = make_uint4(z[4], z[5], 0u, 0u);
+            }
+          }
+        }
+        if (whole) whole_triangle<TW, TH>(job, b, e);
+      }
+    }
+    head[flat] = make_uint2(e.w0, e.w1);
+  }
+  // Wave-uniform loop over the k-th entry of every lane; one atomic per
+  // distinct tile, all groups' leaders in one instruction.
+  for (int k = 0; __ballot(k < e.n); ++k) {
+    const bool has = k < e.n;
+    int pix;
+    const uint32_t tile = has ? entry_tile<TW, TH>(e, k, pix) : 0xffffffffu;
+    int leader, rank, size;
+    group_by_tile(has, tile, leader, rank, size);
+    if (has && rank == 0) atomicAdd(&counter[tile], uint32_t(size));
+  }
+}
+
+template <int TW, int TH>
+__global__ __launch_bounds__(kBlock) void bin_fill_kernel(
+    Jobs J, int64_t total_tris, uint32_t* __restrict__ counter,
+    const uint32_t* __restrict__ offsets, uint64_t* __restrict__ bin_key,
+    uint16_t* __restrict__ bin_pix, uint64_t capacity,
+    const uint2* __restrict__ head, const uint4* __restrict__ frag_z) {
+  const int64_t flat = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  LaneEntries e;
+  uint32_t z[kInlineFrags] = {0u, 0u, 0u, 0u, 0u, 0u};
+  if (flat < total_tris) {
+    const uint2 h = head[flat];
+    e.w0 = h.x;
+    e.w1 = h.y;
+    if (e.w0 != 0) {
+      const dr_raster_job& job = J.table[find_job<false>(J, flat)];
+      e.t = int(flat - job.triangle_begin);
+      e.tile_begin = job.tile_begin;
+      e.ntx = tiles_x(job, TW);
+      if (e.w0 & kHeadFrag) {
+        e.n = int(e.w0 & 7u);
+        const uint4 a = frag_z[2 * flat];
+        z[0] = a.x; z[1] = a.y; z[2] = a.z; z[3] = a.w;
+        if (e.n > 4) {
+          const uint4 c = frag_z[2 * flat + 1];
+          z[4] = c.x; z[5] = c.y;
+        }
+      } else {
+        whole_from_head<TW, TH>(J, flat, e);
       }
     }
   }
-  // Wave-uniform loop over the k-th tile of every lane.
-  for (int k = 0; __ballot(k < n); ++k) {
-    const bool has = k < n;
-    const uint32_t tile =
-        has ? tile0 + uint32_t(k / nx) * uint32_t(ntx) + uint32_t(k % nx)
-            : 0xffffffffu;
-    // Group the lanes by tile with ballots alone (no memory traffic): each
-    // lane learns the lowest lane of its group, its rank in the group and
-    // the group size.
-    uint64_t todo = __ballot(has);
-    int leader = 0, rank = 0, size = 0;
-    while (todo) {
-      const int first = __ffsll(static_cast<unsigned long long>(todo)) - 1;
-      const uint32_t lt = __builtin_amdgcn_readlane(tile, first);
-      const bool mine = has && tile == lt;
-      const uint64_t peers = __ballot(mine);
-      if (mine) {
-        leader = first;
-        rank = __popcll(peers & lanemask_lt());
-        size = __popcll(peers);
-      }
-      todo &= ~peers;
-    }
-    // One atomic per distinct tile, all groups' leaders in one instruction,
-    // so the wave waits for a single round trip per k.
+  for (int k = 0; __ballot(k < e.n); ++k) {
+    const bool has = k < e.n;
+    int pix = 0;
+    uint32_t tile = 0xffffffffu;
+    if (has) tile = entry_tile<TW, TH>(e, k, pix);
+    int leader, rank, size;
+    group_by_tile(has, tile, leader, rank, size);
     uint32_t base = 0;
     if (has && rank == 0) base = atomicAdd(&counter[tile], uint32_t(size));
-    if (kFill) {
-      base = __shfl(base, leader, 64);
-      if (has) {
-        const uint64_t pos = uint64_t(offsets[tile]) + base + uint32_t(rank);
-        if (pos < capacity) bins[pos] = uint32_t(t);
+    base = __shfl(base, leader, 64);
+    if (has) {
+      const uint64_t pos = uint64_t(offsets[tile]) + base + uint32_t(rank);
+      if (pos < capacity) {
+        uint32_t hi = kTriMarker;
+        if (e.w0 & kHeadFrag) {
+          uint32_t zb = z[0];
+#pragma unroll
+          for (int q = 1; q < kInlineFrags; ++q)
+            if (k == q) zb = z[q];
+          hi = z_order(__uint_as_float(zb));
+          bin_pix[pos] = uint16_t(pix);
+        }
+        bin_key[pos] = (uint64_t(hi) << 32) | uint32_t(e.t);
       }
     }
   }
@@ -635,27 +782,84 @@ __device__ __forceinline__ void reference_insert(float* lz, int* lid, int L,
   }
 }
 
+// z of an order key (z_order's inverse; -0 comes back as +0, which compares
+// the same).
+__device__ __forceinline__ float z_from_order(uint32_t o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7fffffffu) : ~o);
+}
+
+// Where a tile's items come from: its bin entries, or -- when the bins
+// overflowed -- every triangle of the job through its count-pass record.
+struct TileItems {
+  bool overflow;
+  uint64_t off;                  // first bin entry
+  const uint64_t* bin_key;
+  const uint16_t* bin_pix;
+  const uint2* head;
+  const uint32_t* frag_z;        // [T][8] words
+};
+
+// Fragment of item i at pixel (ix, iy), p its index in the tile: triangle t
+// and its z, or false when item i has none there.
+__device__ __forceinline__ bool item_fragment(const dr_raster_job& job,
+                                              const TileItems& it, int64_t i,
+                                              int ix, int iy, int p, float sx2,
+                                              float sy2, int cull, int& t,
+                                              float& z) {
+  if (!it.overflow) {
+    const uint64_t key = it.bin_key[it.off + i];
+    t = int(uint32_t(key));
+    const uint32_t hi = uint32_t(key >> 32);
+    if (hi != kTriMarker) {
+      if (int(it.bin_pix[it.off + i]) != p) return false;
+      z = z_from_order(hi);
+      return true;
+    }
+  } else {
+    t = int(i);
+    const int64_t flat = job.triangle_begin + i;
+    const uint2 h = it.head[flat];
+    if (h.x == 0) return false;
+    if (h.x & kHeadFrag) {
+      // A fragment record lists all of the triangle's fragments.
+      const int dx = ix - int(h.y & 0xffffu), dy = iy - int(h.y >> 16);
+      if (dx < 0 || dx >= kSmallSide || dy < 0 || dy >= kSmallSide)
+        return false;
+      const uint32_t want = uint32_t((dy << 2) | dx);
+      const int n = int(h.x & 7u);
+      for (int k = 0; k < n; ++k)
+        if (((h.x >> (3 + 4 * k)) & 15u) == want) {
+          z = __uint_as_float(it.frag_z[8 * flat + k]);
+          return true;
+        }
+      return false;
+    }
+  }
+  Fragment f;
+  if (!fragment_at(job, t, ix, iy, sx2, sy2, cull, f)) return false;
+  z = f.z;
+  return true;
+}
+
 // Block-cooperative; every thread of the block must call it.  Inlined: as a
-// called function its register demand (145 VGPRs) became the whole tile
-// kernel's, three resident waves per SIMD instead of five.
+// called function its register demand became the whole tile kernel's (three
+// resident waves per SIMD instead of five).
 __device__ __forceinline__ void replay_pixel(
-    const dr_raster_job& job, int ix, int iy, float sx2, float sy2, int cull,
-    bool overflow, uint32_t cnt, uint64_t off,
-    const uint32_t* __restrict__ bins, int L, uint64_t* slot, int tp,
-    uint64_t* cand, int* cand_n) {
+    const dr_raster_job& job, const TileItems& it, int64_t items, int ix,
+    int iy, int p, float sx2, float sy2, int cull, int L, uint64_t* slot,
+    int tp, uint64_t* cand, int* cand_n) {
   const uint64_t last = slot[(L - 1) * tp];
   const uint32_t zmax = last == kEmpty ? 0xffffffffu : uint32_t(last >> 32);
-  const int64_t items = overflow ? int64_t(job.num_triangles) : int64_t(cnt);
   if (threadIdx.x == 0) *cand_n = 0;
   __syncthreads();
   for (int64_t i = threadIdx.x; i < items; i += blockDim.x) {
-    const int t = overflow ? int(i) : int(bins[off + i]);
-    Fragment f;
-    if (!fragment_at(job, t, ix, iy, sx2, sy2, cull, f)) continue;
-    if (z_order(f.z) > zmax) continue;
+    int t;
+    float z;
+    if (!item_fragment(job, it, i, ix, iy, p, sx2, sy2, cull, t, z)) continue;
+    if (z_order(z) > zmax) continue;
     const int k = atomicAdd(cand_n, 1);
     if (k < kMaxCandidates)
-      cand[k] = (uint64_t(uint32_t(t)) << 32) | __float_as_uint(f.z);
+      cand[k] = (uint64_t(uint32_t(t)) << 32) | __float_as_uint(z);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -686,13 +890,14 @@ __device__ __forceinline__ void replay_pixel(
         int next = INT32_MAX;
         float next_z = 1.0f;
         for (int64_t i = 0; i < items; ++i) {
-          const int t = overflow ? int(i) : int(bins[off + i]);
+          int t;
+          float z;
+          if (!item_fragment(job, it, i, ix, iy, p, sx2, sy2, cull, t, z))
+            continue;
           if (t <= prev || t >= next) continue;
-          Fragment f;
-          if (!fragment_at(job, t, ix, iy, sx2, sy2, cull, f)) continue;
-          if (z_order(f.z) > zmax) continue;
+          if (z_order(z) > zmax) continue;
           next = t;
-          next_z = f.z;
+          next_z = z;
         }
         if (next == INT32_MAX) break;
         reference_insert(lz, lid, L, next_z, next);
@@ -713,14 +918,18 @@ struct TriRect {
 };
 
 // Rasterizes all bin entries of one tile per block, then resolves the tile.
-// Five waves per SIMD (at most 96 VGPRs): left to itself the compiler
-// keeps more loads in flight per wave at 97-100 VGPRs, four waves per SIMD,
-// which is slower.
+// A fragment entry is inserted into the tile's depth lists as it is (its
+// set-up and edge test ran in the count pass); a whole-triangle entry is
+// set up here and its pixels walked: by its own lane when its clipped rect
+// is small, else queued and walked by a whole wave, one pixel per lane.
+// Five waves per SIMD (at most 96 VGPRs).
 template <int TW, int TH>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
     Jobs J, int L, int cull, const uint32_t* __restrict__ counts,
-    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ bins,
+    const uint32_t* __restrict__ offsets,
+    const uint64_t* __restrict__ bin_key, const uint16_t* __restrict__ bin_pix,
+    const uint2* __restrict__ head, const uint32_t* __restrict__ frag_z,
     uint64_t capacity, bool sparse) {
   constexpr int TP = TW * TH;
   extern __shared__ uint64_t keys[];  // [L][TP]
@@ -745,10 +954,17 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
 
   const uint32_t cnt = counts[flat_tile];
   const uint64_t off = offsets[flat_tile];
-  const bool overflow = off + cnt > capacity;
-  const int64_t items = overflow ? int64_t(job.num_triangles) : int64_t(cnt);
+  TileItems it;
+  it.overflow = off + cnt > capacity;
+  it.off = off;
+  it.bin_key = bin_key;
+  it.bin_pix = bin_pix;
+  it.head = head;
+  it.frag_z = frag_z;
+  const int64_t items =
+      it.overflow ? int64_t(job.num_triangles) : int64_t(cnt);
   const int64_t P = int64_t(W) * H;
-  if (items == 0) {
+  if (cnt == 0) {
     // Empty tile (most of a frame): background everywhere, no LDS.
     for (int p = threadIdx.x; p < TP; p += kBlock) {
       const int ix = x0 + p % TW, iy = y0 + p / TW;
@@ -778,10 +994,42 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
     bool wave_job = false;
     TriRect rect;
     if (i < items) {
-      const int t = overflow ? int(i) : int(bins[off + i]);
+      int t = -1;  // a whole triangle to walk
+      if (!it.overflow) {
+        const uint64_t key = bin_key[off + i];
+        if (uint32_t(key >> 32) != kTriMarker)
+          insert_key(keys, ties, TP, int(bin_pix[off + i]), L, key);
+        else
+          t = int(uint32_t(key));
+      } else {
+        // Overflowed bins: every triangle of the job, through its record
+        // (fragments by pixel; whole triangles by their tile range).
+        const int64_t flat = job.triangle_begin + i;
+        const uint2 h = head[flat];
+        if (h.x & kHeadFrag) {
+          const int n = int(h.x & 7u);
+          for (int k = 0; k < n; ++k) {
+            const uint32_t o = (h.x >> (3 + 4 * k)) & 15u;
+            const int px = int(h.y & 0xffffu) + int(o & 3u);
+            const int py = int(h.y >> 16) + int(o >> 2);
+            if (px < x0 || px >= x1 || py < y0 || py >= y1) continue;
+            const float z = __uint_as_float(frag_z[8 * flat + k]);
+            insert_key(keys, ties, TP, (py - y0) * TW + (px - x0), L,
+                       (uint64_t(z_order(z)) << 32) | uint32_t(i));
+          }
+        } else if (h.x & kHeadRecompute) {
+          t = int(i);
+        } else if (h.x & kHeadTri) {
+          const int64_t d = flat_tile - int64_t(h.y);
+          const int64_t tpr = int64_t((h.x >> 16) & 8191u);
+          if (d >= 0 && d / tpr < int64_t((h.x >> 8) & 255u) &&
+              d % tpr < int64_t(h.x & 255u))
+            t = int(i);
+        }
+      }
       float4 v[3];
       Box b;
-      if (load_triangle(job, t, v) && triangle_bbox(v, W, H, b)) {
+      if (t >= 0 && load_triangle(job, t, v) && triangle_bbox(v, W, H, b)) {
         rect.tri = t;
         rect.x0 = short(max(b.left, x0));
         rect.x1 = short(min(b.right, x1));
@@ -844,8 +1092,8 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
       bits &= bits - 1;
       const int ix = x0 + p % TW, iy = y0 + p / TW;
       if (ix >= x1 || iy >= y1) continue;
-      replay_pixel(job, ix, iy, sx2, sy2, cull, overflow, cnt, off, bins, L,
-                   keys + p, TP, cand, &cand_n);
+      replay_pixel(job, it, items, ix, iy, p, sx2, sy2, cull, L, keys + p, TP,
+                   cand, &cand_n);
     }
   }
 
@@ -887,8 +1135,10 @@ struct Workspace {
   uint32_t* offsets;
   uint32_t* block_sums;
   uint32_t* status;
-  uint32_t* bins;
-  uint2* desc;         // per-triangle tile ranges (count pass -> fill pass)
+  uint64_t* bin_key;   // [capacity] fragment keys / whole-triangle entries
+  uint16_t* bin_pix;   // [capacity] a fragment's pixel in its tile
+  uint2* head;         // [T] count-pass record heads
+  uint4* frag_z;       // [T][2] the records' fragment z
   dr_raster_job* job;  // the single-job entry point's one-entry table
   size_t bytes;
 };
@@ -918,10 +1168,15 @@ Workspace carve(void* base, int64_t total_tiles, uint64_t capacity,
   o += align256(sizeof(uint32_t) * total_tiles);
   w.block_sums = reinterpret_cast<uint32_t*>(p + o);
   o += align256(sizeof(uint32_t) * scan_blocks(total_tiles));
-  w.bins = reinterpret_cast<uint32_t*>(p + o);
-  o += align256(sizeof(uint32_t) * capacity);
-  w.desc = reinterpret_cast<uint2*>(p + o);
-  o += align256(sizeof(uint2) * size_t(total_tris > 0 ? total_tris : 1));
+  const size_t nt = size_t(total_tris > 0 ? total_tris : 1);
+  w.bin_key = reinterpret_cast<uint64_t*>(p + o);
+  o += align256(sizeof(uint64_t) * capacity);
+  w.bin_pix = reinterpret_cast<uint16_t*>(p + o);
+  o += align256(sizeof(uint16_t) * capacity);
+  w.head = reinterpret_cast<uint2*>(p + o);
+  o += align256(sizeof(uint2) * nt);
+  w.frag_z = reinterpret_cast<uint4*>(p + o);
+  o += align256(2 * sizeof(uint4) * nt);
   w.job = reinterpret_cast<dr_raster_job*>(p + o);
   o += align256(sizeof(dr_raster_job));
   w.bytes = o;
@@ -949,8 +1204,8 @@ int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
     DR_CHECK_LAUNCH("raster clear");
     const unsigned tri_blocks = unsigned((total_tris + kBlock - 1) / kBlock);
     if (tri_blocks > 0) {
-      bin_kernel<TW, TH, false><<<tri_blocks, kBlock, 0, s>>>(
-          J, total_tris, cull, w.counts, w.offsets, w.bins, capacity, w.desc);
+      bin_count_kernel<TW, TH><<<tri_blocks, kBlock, 0, s>>>(
+          J, total_tris, cull, w.counts, w.head, w.frag_z);
       DR_CHECK_LAUNCH("raster bin count");
     }
     const int64_t sb = scan_blocks(total_tiles);
@@ -963,15 +1218,17 @@ int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
     // The fill pass reuses counts (zeroed by the local scan) as cursors: the
     // tile kernel needs the final counts, which equal the cursors after it.
     if (tri_blocks > 0) {
-      bin_kernel<TW, TH, true><<<tri_blocks, kBlock, 0, s>>>(
-          J, total_tris, cull, w.counts, w.offsets, w.bins, capacity, w.desc);
+      bin_fill_kernel<TW, TH><<<tri_blocks, kBlock, 0, s>>>(
+          J, total_tris, w.counts, w.offsets, w.bin_key, w.bin_pix, capacity,
+          w.head, w.frag_z);
       DR_CHECK_LAUNCH("raster bin fill");
     }
   }
   if (!run_tiles) return kOk;
   const size_t lds = sizeof(uint64_t) * L * TW * TH;
   raster_tile_kernel<TW, TH><<<unsigned(total_tiles), kBlock, lds, s>>>(
-      J, L, cull, w.counts, w.offsets, w.bins, capacity,
+      J, L, cull, w.counts, w.offsets, w.bin_key, w.bin_pix, w.head,
+      reinterpret_cast<const uint32_t*>(w.frag_z), capacity,
       (flags & DR_RASTER_SPARSE) != 0);
   DR_CHECK_LAUNCH("raster tiles");
   return kOk;
