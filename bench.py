@@ -258,24 +258,34 @@ def cpu_raster_parallel(w, args, ras):
 
 def finite_differences(r, vertices, target, loss_scale, count=64, eps=1e-3,
                        seed=3):
-  """Central differences (eps 1e-3, rasterization frozen) of the fused
-  path's L2 loss for `count` seeded vertices with a non-zero gradient, x 3
-  coordinates, against r.d_vertices[0] from the last step: relative error
-  ||fd - g|| / ||g|| over the set.  Only the pixels a vertex touches change
-  between the two renders, so their float64 difference is the exact local
-  loss difference of the float32 images."""
+  """Central differences (rasterization frozen) of the fused path's L2 loss
+  for `count` seeded vertices with a non-zero gradient, x 3 coordinates,
+  against r.d_vertices[0] from the last step.  Only the pixels a vertex
+  touches change between the two renders, so their float64 difference is
+  the exact local loss difference of the float32 images.
+
+  An entry "crosses the kink" when, between its two renders and the
+  unperturbed one, some pixel's accumulated splat weight of some layer class
+  moves across 1, where the normalisation 1 / (max(w - 1, 0) + 1)
+  (splat.py:168-170) changes branch: the quotient then mixes two branches of
+  the loss.  Needs r built with return_accum_buffers=True.
+
+  Returns the relative error ||fd - g|| / ||g|| over all entries and over
+  the entries that cross no kink, and the number that do."""
   import numpy as np
   import torch
   g = r.d_vertices[0].clone()
+  base_side = r.accum_weights[0] > 1.0
   cand = torch.nonzero(g.abs().sum(1) > 0).squeeze(1).cpu().numpy()
   sel = np.random.default_rng(seed).choice(cand, min(count, cand.size),
                                            replace=False)
   base = vertices.clone()
   tgt = target.double()
-  fd, an = [], []
+  fd, an, kink = [], [], []
   for v in sel:
     for c in range(3):
       losses, pos = [], []
+      crossed = False
       for sgn in (1.0, -1.0):
         vertices.copy_(base)
         vertices[v, c] += sgn * eps
@@ -283,13 +293,22 @@ def finite_differences(r, vertices, target, loss_scale, count=64, eps=1e-3,
         r.transform()
         r.splat_forward(0)
         losses.append(((r.images[0].double() - tgt) ** 2))
+        crossed |= bool(((r.accum_weights[0] > 1.0) != base_side).any())
       d = (losses[0] - losses[1]).sum().item() * loss_scale
       fd.append(d / (pos[0] - pos[1]))
       an.append(float(g[v, c]))
+      kink.append(crossed)
   vertices.copy_(base)
   r.transform()
-  fd, an = np.array(fd), np.array(an)
-  return float(np.linalg.norm(fd - an) / np.linalg.norm(an))
+  r.splat_forward(0)
+  fd, an, kink = np.array(fd), np.array(an), np.array(kink)
+  free = ~kink
+  return dict(
+      rel_err=float(np.linalg.norm(fd - an) / np.linalg.norm(an)),
+      rel_err_kink_free=(float(np.linalg.norm((fd - an)[free]) /
+                               np.linalg.norm(an[free]))
+                         if free.any() else None),
+      entries=int(fd.size), crossing_kink=int(kink.sum()))
 
 
 def grad_check(w, args, dev):
@@ -301,7 +320,8 @@ def grad_check(w, args, dev):
   from oracle import splat_oracle as so
   R, L = args.resolution, args.layers
   r = batch.BatchedSplatRenderer([w["mesh"]], [(0, w["cams_t"][0], R, R)], L,
-                                 loss_scale=1.0 / (R * R * 4))
+                                 loss_scale=1.0 / (R * R * 4),
+                                 return_accum_buffers=True)
   r.targets[0].copy_(w["targets"][0])
   r.step()
   torch.cuda.synchronize()
@@ -344,11 +364,10 @@ def grad_check(w, args, dev):
   plain_image_err64 = rel(plain, out64)
   fused_equals_plain = bool(torch.equal(r.images[0], plain))
 
-  # eps = 1e-3 world units moves a fragment by up to 0.3-0.44 px here, about
-  # 0.6-0.9 sigma of the splat's Gaussian (sigma 0.5 px, splat.py:65-66), so
-  # the central difference carries a large truncation error (it falls as
-  # eps^2: 1e-4 is 0.03 px).  No fragment leaves its pixel cell at either
-  # step (the frozen barycentrics put it at the pixel centre).
+  # Central differences at eps 1e-3 (0.3-0.44 px of fragment motion here,
+  # enough to move neighbouring pixels' accumulated weights across the
+  # normalisation kink at 1, 5 % below the full-coverage weight 1.05), and
+  # at 3e-4 and 1e-4, below which no entry crosses it.
   fd = {str(eps): finite_differences(r, w["mesh"]["vertices"],
                                      w["targets"][0], 1.0 / (R * R * 4),
                                      eps=eps) for eps in (1e-3, 3e-4, 1e-4)}

@@ -58,6 +58,7 @@ class SplatJob(ctypes.Structure):
               ("barycentrics", _c_ptr), ("image", _c_ptr),
               ("target", _c_ptr), ("d_image", _c_ptr),
               ("grad_scratch", _c_ptr), ("d_records", _c_ptr),
+              ("accum", _c_ptr), ("norm", _c_ptr), ("accum_weights", _c_ptr),
               ("num_vertices", _c_int),
               ("num_triangles", _c_int), ("width", _c_int),
               ("height", _c_int), ("tile_begin", _c_i64)]

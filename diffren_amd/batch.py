@@ -50,6 +50,11 @@ class BatchedSplatRenderer:
            float expression as the transform launch, so results are
            identical).
 
+  return_accum_buffers: also keep, per job, the splat's accumulated and
+           normalised fg / surface / bg layers accum[i], norm[i]
+           [3, H, W, C] (render_triangles(return_accum_buffers=True),
+           render.py:130-132) and their weights accum_weights[i] [3, H, W].
+
   After step(): images[i], loss (sum over jobs of loss_scale * |img - tgt|^2),
   d_vertices[m] [V,3] and d_colors[m] [V,C] (views of records[m]).  ids[i]
   holds triangle ids; z[i] / barys[i] are defined under triangles only.
@@ -57,7 +62,7 @@ class BatchedSplatRenderer:
 
   def __init__(self, meshes, jobs, num_layers, tile=None, bin_capacity=0,
                loss_scale=None, autotune_cache=None, streams=1,
-               fused_transform=False):
+               fused_transform=False, return_accum_buffers=False):
     lib = _native.lib()
     self.meshes = meshes
     self.jobs = jobs
@@ -117,6 +122,15 @@ class BatchedSplatRenderer:
                  self.clip_blocks[mi][self.mesh_views[mi].index(i)]
                  for i, (mi, _, _, _) in enumerate(jobs)]
     sf = lib.dr_render_scratch_floats(C)
+    self.accum, self.norm, self.accum_weights = [], [], []
+    if return_accum_buffers:
+      for _, _, w, h in jobs:
+        self.accum.append(torch.zeros((3, h, w, C), dtype=torch.float32,
+                                      device=dev))
+        self.norm.append(torch.zeros((3, h, w, C), dtype=torch.float32,
+                                     device=dev))
+        self.accum_weights.append(torch.zeros((3, h, w), dtype=torch.float32,
+                                              device=dev))
     npix = 0
     for mi, cam, w, h in jobs:
       self.ids.append(torch.empty((self.L, h, w), dtype=torch.int32,
@@ -149,6 +163,19 @@ class BatchedSplatRenderer:
       self._plan_raster(g)
       self._plan_splat(g)
     self.graph = None
+    self._graph_forward_only = False
+    # Bin-overflow watch.  Each group's rasterizer status words (entries
+    # needed, overflow flag) are copied into pinned host memory at the end
+    # of its raster stage, inside a captured graph too.  The next step() or
+    # replay() reads them once the previous step has completed; when a
+    # group's bins overflowed (its tiles then fell back to scanning every
+    # triangle record of their job: correct, slow), the bin capacity grows to
+    # twice the entries needed, the workspaces are re-planned and a captured
+    # graph is captured again.  `regrows` counts these.
+    self._status_host = torch.zeros((len(self.groups), 16), dtype=torch.uint8,
+                                    pin_memory=True)
+    self._status_event = None
+    self.regrows = 0
 
   # -- planning --------------------------------------------------------------
 
@@ -236,6 +263,10 @@ class BatchedSplatRenderer:
       j.d_image = None
       j.grad_scratch = self.scratch[i].data_ptr()
       j.d_records = self.records[mi].data_ptr()
+      if self.accum:
+        j.accum = self.accum[i].data_ptr()
+        j.norm = self.norm[i].data_ptr()
+        j.accum_weights = self.accum_weights[i].data_ptr()
       j.num_vertices = m["vertices"].shape[0]
       j.num_triangles = m["triangles"].shape[0]
       j.width = w
@@ -312,6 +343,8 @@ class BatchedSplatRenderer:
 
   def _group_step(self, g, grad_mode):
     self._rasterize(g)
+    self._status_host[self.groups.index(g)].copy_(g.raster_ws[:16],
+                                                  non_blocking=True)
     self._splat_forward(g, grad_mode)
     if grad_mode:
       self._splat_backward(g)
@@ -329,14 +362,51 @@ class BatchedSplatRenderer:
     for s in self.streams:
       cur.wait_stream(s)
 
+  def _check_overflow(self):
+    """Regrows the bins if the last completed step overflowed them."""
+    ev = self._status_event
+    if ev is None or not ev.query():
+      return
+    self._status_event = None
+    st = self._status_host.view(torch.int32).reshape(len(self.groups), 4)
+    st = st.tolist()
+    if not any(row[1] for row in st):
+      return
+    need = max((row[0] & 0xffffffff) | (row[2] << 32) for row in st)
+    cap = max(int(2 * need) + 4096, int(self.raster_config.bin_capacity))
+    self.raster_config = _native.RasterConfig(
+        self.raster_config.tile_w, self.raster_config.tile_h, cap,
+        self.raster_config.flags, 0)
+    for g in self.groups:
+      self._plan_raster(g)
+    self.regrows += 1
+    if self.graph is not None:
+      self.graph = None
+      self.capture(forward_only=self._graph_forward_only)
+
+  def _mark_step(self):
+    ev = torch.cuda.Event()
+    ev.record()
+    self._status_event = ev
+
   def step(self):
     """One forward + L2 loss + backward over all jobs (enqueue only)."""
+    self._check_overflow()
+    self._enqueue_step()
+    self._mark_step()
+
+  def _enqueue_step(self):
     self.zero_grads()
     self.transform()
     self._run_groups(1)
 
   def render(self):
     """Forward only: fills self.images."""
+    self._check_overflow()
+    self._enqueue_render()
+    self._mark_step()
+
+  def _enqueue_render(self):
     self.transform()
     self._run_groups(0)
 
@@ -345,7 +415,8 @@ class BatchedSplatRenderer:
   def capture(self, warmup=2, forward_only=False):
     """Captures step() (or render() when forward_only) in a HIP graph;
     replay() runs it.  Returns the graph."""
-    fn = self.render if forward_only else self.step
+    fn = self._enqueue_render if forward_only else self._enqueue_step
+    self._graph_forward_only = forward_only
     s = torch.cuda.Stream()
     s.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(s):
@@ -361,9 +432,11 @@ class BatchedSplatRenderer:
     return g
 
   def replay(self):
+    self._check_overflow()
     if self.graph is None:
       self.capture()
     self.graph.replay()
+    self._mark_step()
 
   def bin_stats(self):
     """(entries needed, overflowed) of the last raster launch of each group,

@@ -422,6 +422,17 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
     }
     const float a0 = n[0][C - 1], a1 = n[1][C - 1];
     const int64_t p = int64_t(y) * W + x;
+    if (J.accum || J.norm || J.accum_weights) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (J.accum_weights) J.accum_weights[k * P + p] = acc[k][C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          if (J.accum) J.accum[(k * P + p) * C + c] = acc[k][c];
+          if (J.norm) J.norm[(k * P + p) * C + c] = n[k][c];
+        }
+      }
+    }
     float g[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {

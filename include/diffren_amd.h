@@ -336,6 +336,13 @@ typedef struct dr_splat_job {
                                    camera is set, else cols 0, 1, 3: d clip
                                    x, y, w; cols 4 .. 4+C-1: d colours.
                                    Jobs that share a mesh share it. */
+  float* accum;                 /* [3, H, W, C] accumulated colours of the
+                                   fg / surface / bg layers, or NULL */
+  float* norm;                  /* [3, H, W, C] normalised, or NULL */
+  float* accum_weights;         /* [3, H, W] accumulated weights, or NULL
+                                   (accum and norm are the reference's
+                                   return_accum_buffers outputs,
+                                   splat.py:166-175) */
   int32_t num_vertices;
   int32_t num_triangles;
   int32_t width;

@@ -268,8 +268,11 @@ def test_large_frames_match_per_view(device):
 
 
 def test_vertex_gradient_matches_central_differences(device):
-  # Central differences of the fused path's own loss (eps 1e-3, rasterization
-  # frozen) for 64 seeded vertices x 3 coordinates.
+  # Central differences of the fused path's own loss (rasterization frozen)
+  # for 64 seeded vertices x 3 coordinates, at eps 1e-3 (0.055 px of
+  # fragment motion on this 192x192 frame) and 3e-4.  Entries whose renders
+  # move a pixel's accumulated weight across the normalisation kink at 1 are
+  # counted and held out of the kink-free error.
   import bench
   v, t = scenes.uv_sphere_mesh(60, 120, seed=2)
   rng = np.random.default_rng(4)
@@ -281,16 +284,41 @@ def test_vertex_gradient_matches_central_differences(device):
   cam = torch.as_tensor(scenes.orbit_cameras(1, seed=5)[0], device=device)
   R = 192
   r = batch.BatchedSplatRenderer([mesh], [(0, cam, R, R)], 3,
-                                 loss_scale=1.0 / (R * R * 4))
+                                 loss_scale=1.0 / (R * R * 4),
+                                 return_accum_buffers=True)
   target = torch.as_tensor(rng.uniform(size=(R, R, 4)).astype(np.float32),
                            device=device)
   r.targets[0].copy_(target)
   r.step()
   torch.cuda.synchronize()
-  err = bench.finite_differences(r, mesh["vertices"], target,
-                                 1.0 / (R * R * 4))
-  print(f"finite-difference relative error {err:.3e}")
-  assert err <= 2e-2
+  for eps, tol in ((1e-3, 3e-3), (3e-4, 3e-3)):
+    fd = bench.finite_differences(r, mesh["vertices"], target,
+                                  1.0 / (R * R * 4), eps=eps)
+    print(f"eps {eps}: {fd}")
+    assert fd["crossing_kink"] < fd["entries"] // 4
+    assert fd["rel_err_kink_free"] <= tol
+
+
+def test_fused_accum_buffers_match_plain(device):
+  """return_accum_buffers on the fused path: the accumulated and normalised
+  fg / surface / bg layers equal the plain path's (render.py:130-132), and
+  the accumulated weights equal the alpha accumulation for alpha = 1."""
+  meshes = _meshes(device)
+  jobs = _jobs(meshes)
+  r = batch.BatchedSplatRenderer(meshes, jobs, 3, return_accum_buffers=True)
+  r.render()
+  torch.cuda.synchronize()
+  for i, (mi, cam, w, h) in enumerate(jobs):
+    m = meshes[mi]
+    with torch.no_grad():
+      img, accum, norm = render.render_triangles(
+          m["vertices"], {"c": m["colors"]}, m["triangles"], cam, w, h,
+          lambda a: a["c"], num_layers=3, return_accum_buffers=True)
+    assert torch.equal(r.images[i], img)
+    assert torch.equal(r.accum[i], accum)
+    assert torch.equal(r.norm[i], norm)
+    assert helpers.rel_err(r.accum_weights[i].cpu(),
+                           accum[..., -1].cpu()) <= 1e-6
 
 
 def test_invalid_batches_raise(device):
@@ -354,3 +382,57 @@ def test_jobs_without_gradient_inputs(device):
     torch.cuda.synchronize()
     assert torch.equal(image, want)
     assert loss.item() == 0.0
+
+
+def test_bin_overflow_regrows(device):
+  """A mesh grown past the autotuned bin capacity, in a captured graph:
+  the overflowing replay is still bit-exact (its overflowing tiles read
+  every triangle record of their job), the next replay grows the bins and
+  captures the graph again, and from then on nothing overflows."""
+  v, t = scenes.uv_sphere_mesh(100, 200, seed=3)
+  rng = np.random.default_rng(8)
+  col = np.concatenate([rng.uniform(0, 1, (v.shape[0], 3)),
+                        np.ones((v.shape[0], 1))], 1).astype(np.float32)
+  # Tuned while the mesh is small on screen (0.25 x: a few thousand
+  # fragments), rendered at full size (about 20k fragments, each a bin
+  # entry of a small triangle).
+  full = torch.as_tensor(v, device=device)
+  mesh = dict(vertices=full * 0.25,
+              triangles=torch.as_tensor(t, device=device),
+              colors=torch.as_tensor(col, device=device))
+  cam_np = scenes.orbit_cameras(1, seed=2)[0]
+  cam = torch.as_tensor(cam_np, device=device)
+  R, L = 256, 3
+  r = batch.BatchedSplatRenderer([mesh], [(0, cam, R, R)], L, tile="auto")
+  cap = r.raster_config.bin_capacity
+  r.capture(forward_only=True)
+  r.replay()
+  torch.cuda.synchronize()
+  need0, over0 = r.bin_stats()
+  assert not over0 and need0 <= cap
+
+  def check_ids():
+    clip = so.transform(cam_np, mesh["vertices"].cpu().numpy(),
+                        torch.float32).numpy()
+    want, _, _ = raster_oracle.rasterize(clip, t, R, R, L, 1)
+    assert np.array_equal(r.ids[0].cpu().numpy(), want)
+
+  mesh["vertices"].copy_(full)
+  r.replay()
+  torch.cuda.synchronize()
+  need1, over1 = r.bin_stats()
+  assert over1 and need1 > cap, (need0, need1, cap)
+  check_ids()
+  assert r.regrows == 0
+  r.replay()
+  torch.cuda.synchronize()
+  assert r.regrows == 1
+  assert r.raster_config.bin_capacity >= 2 * need1
+  need2, over2 = r.bin_stats()
+  assert not over2 and need2 == need1
+  check_ids()
+  want = r.images[0].clone()
+  r.replay()
+  torch.cuda.synchronize()
+  assert r.regrows == 1
+  assert torch.equal(r.images[0], want)
