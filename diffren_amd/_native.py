@@ -21,6 +21,7 @@ _c_size = ctypes.c_size_t
 RASTER_SPARSE = 1
 RASTER_BINS_ONLY = 2
 RASTER_TILES_ONLY = 4
+RENDER_ACCUM_BUFFERS = 4
 MLP_ALBEDO_MODULATE = 1
 PAD_CONSTANT = 0
 PAD_EDGE = 1
@@ -58,7 +59,8 @@ class SplatJob(ctypes.Structure):
               ("barycentrics", _c_ptr), ("image", _c_ptr),
               ("target", _c_ptr), ("d_image", _c_ptr),
               ("grad_scratch", _c_ptr), ("d_records", _c_ptr),
-              ("accum", _c_ptr), ("norm", _c_ptr), ("accum_weights", _c_ptr),
+              ("accum", _c_ptr), ("norm", _c_ptr), ("d_camera", _c_ptr),
+              ("accum_weights", _c_ptr),
               ("num_vertices", _c_int),
               ("num_triangles", _c_int), ("width", _c_int),
               ("height", _c_int), ("tile_begin", _c_i64)]

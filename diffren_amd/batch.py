@@ -55,6 +55,11 @@ class BatchedSplatRenderer:
            [3, H, W, C] (render_triangles(return_accum_buffers=True),
            render.py:130-132) and their weights accum_weights[i] [3, H, W].
 
+  camera_grads: also accumulate d loss / d camera of every job into
+           d_cameras[i] [4, 4] (through the splat's transform, splat.py:
+           209-212; rasterization held fixed as rasterize_triangles_xla.py:
+           51-52 holds it).
+
   After step(): images[i], loss (sum over jobs of loss_scale * |img - tgt|^2),
   d_vertices[m] [V,3] and d_colors[m] [V,C] (views of records[m]).  ids[i]
   holds triangle ids; z[i] / barys[i] are defined under triangles only.
@@ -62,7 +67,8 @@ class BatchedSplatRenderer:
 
   def __init__(self, meshes, jobs, num_layers, tile=None, bin_capacity=0,
                loss_scale=None, autotune_cache=None, streams=1,
-               fused_transform=False, return_accum_buffers=False):
+               fused_transform=False, return_accum_buffers=False,
+               camera_grads=False):
     lib = _native.lib()
     self.meshes = meshes
     self.jobs = jobs
@@ -96,7 +102,10 @@ class BatchedSplatRenderer:
     # cols 4..4+C-1 (one contiguous record per vertex for grouped atomics).
     R = lib.dr_render_record_size(C)
     if R <= 0:
-      raise ValueError(f"unsupported channel count {C}")
+      raise ValueError(
+          f"the fused path takes 1-16, 20, 24 or 28 colour channels, not {C}:"
+          " pad the colours with zero channels before the alpha channel, or "
+          "render through render.render_triangles, which takes any count")
     self.records = [torch.zeros((m["vertices"].shape[0], R),
                                 dtype=torch.float32, device=dev)
                     for m in meshes]
@@ -122,6 +131,8 @@ class BatchedSplatRenderer:
                  self.clip_blocks[mi][self.mesh_views[mi].index(i)]
                  for i, (mi, _, _, _) in enumerate(jobs)]
     sf = lib.dr_render_scratch_floats(C)
+    self.d_cameras = (torch.zeros((len(jobs), 4, 4), dtype=torch.float32,
+                                  device=dev) if camera_grads else None)
     self.accum, self.norm, self.accum_weights = [], [], []
     if return_accum_buffers:
       for _, _, w, h in jobs:
@@ -248,10 +259,11 @@ class BatchedSplatRenderer:
       mi, _, w, h = self.jobs[i]
       m = self.meshes[mi]
       j = table[k]
-      if self.fused_transform:
-        j.positions = m["vertices"].data_ptr()
-      else:
+      j.positions = m["vertices"].data_ptr()
+      if not self.fused_transform:
         j.clip_vertices = self.clip[i].data_ptr()
+      if self.d_cameras is not None:
+        j.d_camera = self.d_cameras[i].data_ptr()
       j.triangles = m["triangles"].data_ptr()
       j.colors = m["colors"].data_ptr()
       j.camera = self.cameras[i].data_ptr()
@@ -313,7 +325,9 @@ class BatchedSplatRenderer:
     _native.check(
         _native.lib().dr_render_splat_forward(
             _native.ptr(g.splat_table), len(g.indices), g.splat_tiles, self.L,
-            self.C, grad_mode, self.loss_scale, _native.ptr(self.loss),
+            self.C,
+            grad_mode | (_native.RENDER_ACCUM_BUFFERS if self.accum else 0),
+            self.loss_scale, _native.ptr(self.loss),
             _native.ptr(g.splat_ws), g.splat_ws.numel(),
             _native.stream_handle()), "dr_render_splat_forward")
 
@@ -339,6 +353,8 @@ class BatchedSplatRenderer:
     # Fill kernels, not memsets, so the step can be graph-captured.
     for t in self.records:
       t.zero_()
+    if self.d_cameras is not None:
+      self.d_cameras.zero_()
     self.loss.zero_()
 
   def _group_step(self, g, grad_mode):

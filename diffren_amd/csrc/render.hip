@@ -44,7 +44,6 @@ namespace {
 constexpr int kBX = 64, kBY = 4;
 constexpr int kRX = kBX + 2, kRY = kBY + 2, kRegion = kRX * kRY;
 constexpr int kMaxLayers = 8;
-constexpr int kMaxChannels = 16;
 
 // Per-pixel gradient scratch written by the target pass for the source pass,
 // kRecord floats per pixel in two planes of the job's buffer:
@@ -229,9 +228,10 @@ __device__ __forceinline__ float norm_scale(float w) {
 // where the rasterizer stored a triangle).  Staging a single layer's splats
 // keeps the block's LDS small enough for seven resident blocks per CU.
 // Six resident waves per SIMD fit without spilling for up to 4 channels.
-template <int C>
+template <int C, bool kAccum>
 __global__ __launch_bounds__(kBX* kBY)
-__attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
+__attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : (C <= 12 ? 4 : (C <= 16 ? 3 : 2)))))
+void target_kernel(
     const dr_splat_job* __restrict__ jobs, int num_jobs, int L, float coef,
     int grad_mode, float loss_scale, float* __restrict__ loss_partials) {
   // float4-aligned, so a C % 4 == 0 colour row is read and written as
@@ -422,7 +422,7 @@ __attribute__((amdgpu_waves_per_eu(C <= 4 ? 6 : 4))) void target_kernel(
     }
     const float a0 = n[0][C - 1], a1 = n[1][C - 1];
     const int64_t p = int64_t(y) * W + x;
-    if (J.accum || J.norm || J.accum_weights) {
+    if (kAccum && (J.accum || J.norm || J.accum_weights)) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         if (J.accum_weights) J.accum_weights[k * P + p] = acc[k][C];
@@ -620,7 +620,9 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   int* hrow = hash_row[wave];
   // Nothing to differentiate into, or no gradient scratch to read: the job
   // contributes nothing (one job per block, so the whole block leaves).
-  if (J.d_records == nullptr || J.grad_scratch == nullptr) return;
+  if ((J.d_records == nullptr && J.d_camera == nullptr) ||
+      J.grad_scratch == nullptr)
+    return;
   const bool inside = x < W && y < H;
   const int64_t P = int64_t(W) * H;
   const int64_t p = int64_t(y) * W + x;
@@ -693,6 +695,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     Corner k = k0;
     const bool frag = l == 0 ? frag0 : any && corners(J, l * P + p, k);
     float base[4 + C];
+    float dclip[3] = {0.0f, 0.0f, 0.0f};  // d clip x, y, w
     if (frag) {
       float sx, sy, cx, cy, cw;
       splat_position<false>(J, k, sx, sy, cx, cy, cw);
@@ -768,6 +771,33 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
       for (int c = 0; c < 4; ++c) base[c] = dp[c];
 #pragma unroll
       for (int c = 0; c < C; ++c) base[4 + c] = dcol[c];
+      dclip[0] = dcx;
+      dclip[1] = dcy;
+      dclip[2] = dcw;
+    }
+    if (J.d_camera) {
+      // clip = M [p, 1] with p the fragment's interpolated world point (the
+      // barycentrics interpolate clip coordinates linearly), so
+      // d M[i][j] = sum over fragments of d clip_i [p, 1]_j, rows x, y, w:
+      // a wave sum per layer, then one atomic per entry.
+      float ph[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (frag) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          ph[j] = interp3(k, ldg(J.positions + 3 * int64_t(k.v[0]) + j),
+                          ldg(J.positions + 3 * int64_t(k.v[1]) + j),
+                          ldg(J.positions + 3 * int64_t(k.v[2]) + j));
+        ph[3] = (k.b[0] + k.b[1]) + k.b[2];
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          float d = dclip[i] * ph[j];
+          for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off);
+          if (lane == 0 && d != 0.0f)
+            gatomic_add(J.d_camera + (i < 2 ? 4 * i : 12) + j, d);
+        }
     }
     // Wave-local dedupe into compact LDS rows (one per distinct vertex, in
     // the order first occurrences claim them), then a grouped flush.
@@ -871,18 +901,31 @@ float splat_coef() {
   return float(1.05 / sum);
 }
 
-template <int C>
-int launch_target(const dr_splat_job* jobs, int n, int64_t tiles, int L,
-                  int grad_mode, float loss_scale, float* loss,
-                  float* partials, hipStream_t s) {
+template <int C, bool kAccum>
+int launch_target_kernel(const dr_splat_job* jobs, int n, int64_t tiles,
+                         int L, int grad_mode, float loss_scale,
+                         float* partials, hipStream_t s) {
   const size_t lds = target_lds_bytes(L, C);
   if (lds > 65536) {
     DR_CHECK_HIP(hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&target_kernel<C>),
+        reinterpret_cast<const void*>(&target_kernel<C, kAccum>),
         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
   }
-  target_kernel<C><<<unsigned(tiles), dim3(kBX, kBY), lds, s>>>(
+  target_kernel<C, kAccum><<<unsigned(tiles), dim3(kBX, kBY), lds, s>>>(
       jobs, n, L, splat_coef(), grad_mode, loss_scale, partials);
+  return kOk;
+}
+
+template <int C>
+int launch_target(const dr_splat_job* jobs, int n, int64_t tiles, int L,
+                  int grad_mode, bool accum, float loss_scale, float* loss,
+                  float* partials, hipStream_t s) {
+  const int rc =
+      accum ? launch_target_kernel<C, true>(jobs, n, tiles, L, grad_mode,
+                                            loss_scale, partials, s)
+            : launch_target_kernel<C, false>(jobs, n, tiles, L, grad_mode,
+                                             loss_scale, partials, s);
+  if (rc != kOk) return rc;
   DR_CHECK_LAUNCH("render splat target pass");
   if (grad_mode == 1 && loss) {
     loss_reduce_kernel<<<1, 1024, 0, s>>>(partials, tiles, loss);
@@ -900,6 +943,8 @@ int launch_source(const dr_splat_job* jobs, int n, int64_t tiles, int L,
   return kOk;
 }
 
+// Channel counts the fused kernels are instantiated for: 1-16, 20, 24, 28
+// (a vertex gradient record of 4 + C floats fits 32).
 #define DR_RENDER_SWITCH(C, CALL) \
   switch (C) {                    \
     case 1: return CALL(1);       \
@@ -910,16 +955,29 @@ int launch_source(const dr_splat_job* jobs, int n, int64_t tiles, int L,
     case 6: return CALL(6);       \
     case 7: return CALL(7);       \
     case 8: return CALL(8);       \
+    case 9: return CALL(9);       \
+    case 10: return CALL(10);     \
+    case 11: return CALL(11);     \
     case 12: return CALL(12);     \
+    case 13: return CALL(13);     \
+    case 14: return CALL(14);     \
+    case 15: return CALL(15);     \
     case 16: return CALL(16);     \
+    case 20: return CALL(20);     \
+    case 24: return CALL(24);     \
+    case 28: return CALL(28);     \
   }
+
+bool render_channels_ok(int c) {
+  return (c >= 1 && c <= 16) || c == 20 || c == 24 || c == 28;
+}
 
 int check(int n, int64_t tiles, int L, int C) {
   DR_CHECK_ARG(n > 0, "need at least one job");
   DR_CHECK_ARG(tiles > 0 && tiles < (int64_t(1) << 31), "bad tile count");
   DR_CHECK_ARG(L >= 1 && L <= kMaxLayers, "num_layers must be in [1, 8]");
-  DR_CHECK_ARG(C >= 1 && C <= kMaxChannels && (C <= 8 || C == 12 || C == 16),
-               "fused render supports 1-8, 12 or 16 channels");
+  DR_CHECK_ARG(render_channels_ok(C),
+               "fused render supports 1-16, 20, 24 or 28 channels");
   DR_CHECK_ARG(target_lds_bytes(L, C) <= 160 * 1024,
                "layers x channels exceed the LDS stage");
   return kOk;
@@ -928,14 +986,6 @@ int check(int n, int64_t tiles, int L, int C) {
 }  // namespace
 }  // namespace dr
 
-namespace dr {
-namespace {
-// Channel counts the fused kernels are instantiated for.
-bool render_channels_ok(int c) {
-  return (c >= 1 && c <= 8) || c == 12 || c == 16;
-}
-}  // namespace
-}  // namespace dr
 
 extern "C" {
 
@@ -943,7 +993,7 @@ int dr_splat_jobs_prepare(dr_splat_job* jobs, int32_t n, int32_t channels,
                           int64_t* total_tiles) {
   DR_CHECK_ARG(jobs != nullptr && n > 0, "need at least one job");
   DR_CHECK_ARG(dr::render_channels_ok(channels),
-               "unsupported channel count (1-8, 12 or 16)");
+               "unsupported channel count (1-16, 20, 24 or 28)");
   int64_t tiles = 0;
   for (int i = 0; i < n; ++i) {
     dr_splat_job& j = jobs[i];
@@ -960,6 +1010,8 @@ int dr_splat_jobs_prepare(dr_splat_job* jobs, int32_t n, int32_t channels,
                  "clip_vertices must be 16-byte aligned");
     DR_CHECK_ARG((reinterpret_cast<uintptr_t>(j.grad_scratch) & 15) == 0,
                  "grad_scratch must be 16-byte aligned");
+    DR_CHECK_ARG(j.d_camera == nullptr || (j.camera && j.positions),
+                 "d_camera needs camera and positions");
     j.tile_begin = tiles;
     tiles += int64_t((j.width + dr::kBX - 1) / dr::kBX) *
              ((j.height + dr::kBY - 1) / dr::kBY);
@@ -990,7 +1042,10 @@ int dr_render_splat_forward(const dr_splat_job* jobs_device, int32_t num_jobs,
   int rc = dr::check(num_jobs, total_tiles, num_layers, channels);
   if (rc != DR_OK) return rc;
   DR_CHECK_ARG(jobs_device != nullptr, "jobs is null");
-  DR_CHECK_ARG(grad_mode >= 0 && grad_mode <= 2, "grad_mode must be 0, 1, 2");
+  const bool accum = (grad_mode & DR_RENDER_ACCUM_BUFFERS) != 0;
+  grad_mode &= ~DR_RENDER_ACCUM_BUFFERS;
+  DR_CHECK_ARG(grad_mode >= 0 && grad_mode <= 2,
+               "grad_mode must be 0, 1, 2 (| DR_RENDER_ACCUM_BUFFERS)");
   if (grad_mode == 1) {
     DR_CHECK_ARG(workspace != nullptr, "workspace is null");
     if (workspace_bytes < dr_render_splat_workspace_size(total_tiles))
@@ -1000,7 +1055,7 @@ int dr_render_splat_forward(const dr_splat_job* jobs_device, int32_t num_jobs,
   float* partials = static_cast<float*>(workspace);
 #define DR_CALL(CC)                                                     \
   dr::launch_target<CC>(jobs_device, num_jobs, total_tiles, num_layers, \
-                        grad_mode, loss_scale, loss, partials, s)
+                        grad_mode, accum, loss_scale, loss, partials, s)
   DR_RENDER_SWITCH(channels, DR_CALL)
 #undef DR_CALL
   return dr::fail(dr::kUnsupported, "unsupported channel count");

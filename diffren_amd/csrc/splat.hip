@@ -33,7 +33,6 @@ namespace {
 
 constexpr int kBX = 64, kBY = 4;
 constexpr int kMaxLayers = 8;
-constexpr int kMaxChannels = 16;
 
 struct SplatFrame {
   const float* xyz;    // [L, H, W, 3] viewport x, y (pixels) and z in [0, 1]
@@ -334,13 +333,23 @@ int launch_bwd(const SplatFrame& f, const float* d_out, float* G,
     case 14: return CALL(14);    \
     case 15: return CALL(15);    \
     case 16: return CALL(16);    \
+    case 20: return CALL(20);    \
+    case 24: return CALL(24);    \
+    case 28: return CALL(28);    \
+    case 32: return CALL(32);    \
   }
+
+// Channel counts the kernels are instantiated for; callers pad other counts
+// up to one of them (or split them), see splat.py.
+bool splat_channels_ok(int c) {
+  return (c >= 1 && c <= 16) || c == 20 || c == 24 || c == 28 || c == 32;
+}
 
 int check_frame(const SplatFrame& f, int C) {
   DR_CHECK_ARG(f.L >= 1 && f.L <= kMaxLayers, "num_layers must be in [1, 8]");
   DR_CHECK_ARG(f.H > 0 && f.W > 0, "image size must be > 0");
-  DR_CHECK_ARG(C >= 1 && C <= kMaxChannels,
-               "splat channel count must be in [1, 16]");
+  DR_CHECK_ARG(splat_channels_ok(C),
+               "splat channel count must be 1-16, 20, 24, 28 or 32");
   DR_CHECK_ARG(f.xyz && f.color && f.mask, "null input pointer");
   return kOk;
 }

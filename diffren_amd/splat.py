@@ -59,18 +59,66 @@ class _Splat(torch.autograd.Function):
     return d_xyz, d_color, None, None
 
 
+# Channel counts the splat kernels are instantiated for.
+KERNEL_CHANNELS = tuple(range(1, 17)) + (20, 24, 28, 32)
+
+
+def _with_alpha_last(color, k):
+  """color [..., c] padded to k channels with zeros before the last (alpha)
+  channel: the splat treats every channel independently except the last,
+  which `over` uses as alpha for all of them (composite.py:33-37), so the
+  padding changes no other channel and its outputs are dropped."""
+  c = color.shape[-1]
+  if k == c:
+    return color
+  zeros = color.new_zeros(color.shape[:-1] + (k - c,))
+  return torch.cat([color[..., :-1], zeros, color[..., -1:]], -1)
+
+
+def _splat_kernel(xyz, color, mask, want_buffers):
+  """One kernel call for any channel count up to 32 (padded)."""
+  c = color.shape[-1]
+  k = next(n for n in KERNEL_CHANNELS if n >= c)
+  out, accum, norm = _Splat.apply(xyz, _with_alpha_last(color, k), mask,
+                                  want_buffers)
+  if k == c:
+    return out, accum, norm
+  keep = lambda t: None if t is None else torch.cat(
+      [t[..., :c - 1], t[..., -1:]], -1)
+  return keep(out), keep(accum), keep(norm)
+
+
 def splat_at_pixel_centers(xyz_layers, color_layers, foreground_mask,
                            return_buffers=True):
-  """splat.py:26-175.  xyz [L,H,W,3], color [L,H,W,C], mask [L,H,W,1]."""
+  """splat.py:26-175.  xyz [L,H,W,3], color [L,H,W,C], mask [L,H,W,1].
+
+  Any channel count: counts the kernels are not instantiated for are padded
+  up to the next one; above 32 the colour channels are splatted in groups of
+  31, each with the alpha channel, whose outputs come from the first group
+  (so the alpha gradient, a sum over all channels, is the sum of the
+  groups')."""
   if xyz_layers.dim() != 4 or xyz_layers.shape[-1] != 3:
     raise ValueError("xyz_layers must be [L, H, W, 3]")
   if color_layers.dim() != 4 or color_layers.shape[:3] != xyz_layers.shape[:3]:
     raise ValueError("color_layers must be [L, H, W, C] matching xyz_layers")
   _native.require_device(xyz_layers, color_layers, foreground_mask)
   mask = foreground_mask.reshape(xyz_layers.shape[:3])
-  out, accum, norm = _Splat.apply(xyz_layers, color_layers, mask,
-                                  return_buffers)
-  return out, accum, norm
+  c = color_layers.shape[-1]
+  if c <= KERNEL_CHANNELS[-1]:
+    return _splat_kernel(xyz_layers, color_layers, mask, return_buffers)
+  alpha = color_layers[..., -1:]
+  parts = []
+  for c0 in range(0, c - 1, 31):
+    group = torch.cat([color_layers[..., c0:min(c0 + 31, c - 1)], alpha], -1)
+    parts.append(_splat_kernel(xyz_layers, group, mask, return_buffers))
+
+  def join(i):
+    if parts[0][i] is None:
+      return None
+    return torch.cat([p[i][..., :-1] for p in parts] + [parts[0][i][..., -1:]],
+                     -1)
+
+  return join(0), join(1), join(2)
 
 
 def splat_shaded_pixels(vertices, camera_matrices, rasterized_fb,

@@ -285,7 +285,9 @@ float dr_splat_coefficient(void);
  * (diffren/jax/internal/splat.py:26-175).
  *   xyz [L,H,W,3] viewport positions, color [L,H,W,C], mask [L,H,W]
  *   out [H,W,C]; optional (NULL to skip): accum [3,H,W,C], norm [3,H,W,C],
- *   accum_weights [3,H,W].  C in [1, 16], L in [1, 8]. */
+ *   accum_weights [3,H,W].  C one of 1-16, 20, 24, 28, 32 (callers pad
+ *   other counts with zero channels before the alpha channel, or split
+ *   them), L in [1, 8]. */
 int dr_splat_fwd(const float* xyz, const float* color, const float* mask,
                  int32_t num_layers, int32_t height, int32_t width,
                  int32_t channels, float* out, float* accum, float* norm,
@@ -339,6 +341,13 @@ typedef struct dr_splat_job {
   float* accum;                 /* [3, H, W, C] accumulated colours of the
                                    fg / surface / bg layers, or NULL */
   float* norm;                  /* [3, H, W, C] normalised, or NULL */
+  float* d_camera;              /* [4, 4] accumulated gradient of the loss
+                                   with respect to `camera` through the
+                                   splat's transform (splat.py:209-212;
+                                   rows x, y, w; the rasterization is held
+                                   fixed, rasterize_triangles_xla.py:51-52),
+                                   or NULL; needs camera and positions.
+                                   Caller clears. */
   float* accum_weights;         /* [3, H, W] accumulated weights, or NULL
                                    (accum and norm are the reference's
                                    return_accum_buffers outputs,
@@ -355,12 +364,14 @@ int dr_splat_jobs_prepare(dr_splat_job* jobs_host, int32_t num_jobs,
                           int32_t channels, int64_t* total_tiles);
 
 /* Floats per pixel of a job's grad_scratch (0 if the channel count is not
- * one of 1-8, 12, 16). */
+ * one of 1-16, 20, 24, 28). */
 int32_t dr_render_scratch_floats(int32_t channels);
 
 /* Floats per vertex gradient record for `channels` colour channels (8, 16
  * or 32; 0 if unsupported). */
 int32_t dr_render_record_size(int32_t channels);
+
+enum { DR_RENDER_ACCUM_BUFFERS = 4 };
 
 /* Workspace bytes of dr_render_splat_forward (per-tile loss partials). */
 size_t dr_render_splat_workspace_size(int64_t total_tiles);
@@ -372,7 +383,9 @@ size_t dr_render_splat_workspace_size(int64_t total_tiles);
  * loss_scale added to *loss (device float, caller clears; summed per tile,
  * then in a fixed order, so it is deterministic) and its gradient
  * 2 (image - target) loss_scale pushed into grad_scratch.  2: the given
- * d_image pushed into grad_scratch.  channels 1-8, 12 or 16.  The workspace
+ * d_image pushed into grad_scratch.  grad_mode | DR_RENDER_ACCUM_BUFFERS
+ * also writes the jobs' accum / norm / accum_weights outputs (ignored
+ * without it).  channels 1-16, 20, 24 or 28.  The workspace
  * is needed for grad_mode 1 only.  A job whose target (mode 1) or d_image
  * (mode 2) or grad_scratch is NULL gets its image only (no loss, no
  * gradient).

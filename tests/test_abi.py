@@ -94,10 +94,10 @@ def test_struct_layouts_match_header(tmp_path):
 
 def test_render_channel_counts_and_job_checks():
   lib = _native.lib()
-  for c in (1, 4, 8, 12, 16):
+  for c in (1, 4, 8, 9, 12, 13, 16, 20, 24, 28):
     assert lib.dr_render_record_size(c) > 0
     assert lib.dr_render_scratch_floats(c) > 0
-  for c in (0, 9, 11, 13, 17):
+  for c in (0, 17, 21, 29, 32):
     assert lib.dr_render_record_size(c) == 0
     assert lib.dr_render_scratch_floats(c) == 0
   job = (_native.SplatJob * 1)()
@@ -109,7 +109,7 @@ def test_render_channel_counts_and_job_checks():
   tiles = ctypes.c_int64()
   assert lib.dr_splat_jobs_prepare(job, 1, 4, ctypes.byref(tiles)) == 0
   assert tiles.value == 2 * 1  # 64x4 tiles: one column, two rows
-  assert lib.dr_splat_jobs_prepare(job, 1, 9, ctypes.byref(tiles)) != 0
+  assert lib.dr_splat_jobs_prepare(job, 1, 17, ctypes.byref(tiles)) != 0
   assert b"channel" in lib.dr_last_error()
   job[0].grad_scratch = 260
   assert lib.dr_splat_jobs_prepare(job, 1, 4, ctypes.byref(tiles)) != 0

@@ -160,11 +160,13 @@ def test_fused_transform_matches_clip_buffers(streams, device):
     torch.ones(1000, device=device).sum().item()
 
 
-@pytest.mark.parametrize("channels", [1, 12])
+@pytest.mark.parametrize("channels", [1, 9, 12, 15, 28])
 def test_fused_channel_counts(channels, device):
   """The fused path at other channel counts (12: normals + 8 features +
-  alpha, the forward-only workload's pass-through shader) vs the unfused
-  path, forward and backward; forward-only graph replays too."""
+  alpha, the forward-only workload's pass-through shader; 15: the
+  interpolated channels of that workload with the clip xyzw; 28: the
+  largest) vs the unfused path, forward and backward; forward-only graph
+  replays too."""
   meshes = _meshes(device)
   gen = torch.Generator(device="cpu").manual_seed(channels)
   for m in meshes:
@@ -319,6 +321,40 @@ def test_fused_accum_buffers_match_plain(device):
     assert torch.equal(r.norm[i], norm)
     assert helpers.rel_err(r.accum_weights[i].cpu(),
                            accum[..., -1].cpu()) <= 1e-6
+
+
+def test_camera_gradient(device):
+  """d loss / d camera on the fused path (through the splat's transform,
+  splat.py:209-212; rasterization held fixed) against the plain path's
+  autograd and the float64 oracle."""
+  meshes = _meshes(device)
+  jobs = _jobs(meshes)
+  r = batch.BatchedSplatRenderer(meshes, jobs, 3, camera_grads=True)
+  gen = torch.Generator(device="cpu").manual_seed(7)
+  for t in r.targets:
+    t.copy_(torch.rand(t.shape, generator=gen))
+  r.step()
+  torch.cuda.synchronize()
+  for i, (mi, cam, w, h) in enumerate(jobs):
+    m = meshes[mi]
+    cc = cam.clone().requires_grad_(True)
+    img = render.render_triangles(m["vertices"], {"c": m["colors"]},
+                                  m["triangles"], cc, w, h, lambda a: a["c"],
+                                  num_layers=3)
+    (((img - r.targets[i]) ** 2).sum() * r.loss_scale).backward()
+    assert helpers.rel_err(r.d_cameras[i].cpu(), cc.grad.cpu()) <= 1e-4
+    s = m["scene"]
+    cam_np = cam.cpu().numpy()
+    clip32 = so.transform(cam_np, s["vertices"], torch.float32).numpy()
+    ids, _, b = raster_oracle.rasterize(clip32, s["triangles"], w, h, 3, 1)
+    c64 = torch.as_tensor(cam_np, dtype=torch.float64).requires_grad_(True)
+    o = so.render_splat(torch.as_tensor(s["vertices"], dtype=torch.float64),
+                        torch.as_tensor(s["colors"], dtype=torch.float64),
+                        c64, ids, b, s["triangles"])
+    ((o - r.targets[i].cpu().double()) ** 2).sum().mul(r.loss_scale).backward()
+    assert helpers.rel_err(r.d_cameras[i].cpu(), c64.grad) <= 1e-4
+    # z row: the splat takes z from the rasterizer, so it gets no gradient.
+    assert r.d_cameras[i][2].abs().max().item() == 0.0
 
 
 def test_invalid_batches_raise(device):

@@ -95,8 +95,11 @@ def _splat_inputs(seed, w, h, layers, channels):
   return xyz.astype(np.float32), color, mask
 
 
+# 17 and 21: padded to the next instantiated count; 40 and 70: split into
+# groups of 31 colour channels plus alpha.
 @pytest.mark.parametrize("layers,channels", [(1, 4), (2, 4), (3, 4), (3, 12),
-                                             (4, 2)])
+                                             (4, 2), (2, 17), (3, 21), (2, 40),
+                                             (1, 70)])
 def test_splat_forward(layers, channels, device):
   xyz, color, mask = _splat_inputs(layers + channels, 80, 72, layers,
                                    channels)
@@ -111,7 +114,8 @@ def test_splat_forward(layers, channels, device):
   assert helpers.rel_err(norm.cpu(), w_norm) <= FWD_TOL
 
 
-@pytest.mark.parametrize("layers,channels", [(1, 4), (3, 4), (3, 12)])
+@pytest.mark.parametrize("layers,channels", [(1, 4), (3, 4), (3, 12),
+                                             (2, 17), (2, 40), (1, 70)])
 def test_splat_backward_vs_float64_autodiff(layers, channels, device):
   xyz, color, mask = _splat_inputs(7 * layers + channels, 48, 40, layers,
                                    channels)
