@@ -289,18 +289,25 @@ void target_kernel(
   // word t / 4): the same bytes the gradient scratch records.  A region
   // with no fragment (most of a frame) splats nothing and writes no
   // gradient record, so it needs none.
-  bool region_live = false;
-  for (int l = 0; l < L; ++l) region_live |= layer_live[l] != 0;
+  // Live layers form a prefix (a pixel has a layer-l fragment only if it
+  // has one at every layer before l).  Past the first empty layer every
+  // region pixel has z = 1 at every layer, and further such layers can never
+  // win the surface index's strict-< argmins against it, so the index is
+  // computed over the live layers plus one empty layer.
+  int live = 0;
+  for (int l = 0; l < L; ++l) live += layer_live[l] != 0;
+  const bool region_live = live > 0;
+  const int Ls = min(L, live + 1);
   uint32_t sw[3] = {0u, 0u, 0u};
   if (in_image && region_live) {
     float zT[kMaxLayers], zS[kMaxLayers];
-    for (int l = 0; l < L; ++l) zT[l] = sz[l * R + rT];
+    for (int l = 0; l < Ls; ++l) zT[l] = sz[l * R + rT];
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
       const int u = t / 3 - 1, v = t % 3 - 1;
       const int rS = rT - v * kRX - u;
-      for (int l = 0; l < L; ++l) zS[l] = sz[l * R + rS];
-      sw[t >> 2] |= (uint32_t(surface_index(zS, zT, L)) & 0xffu)
+      for (int l = 0; l < Ls; ++l) zS[l] = sz[l * R + rS];
+      sw[t >> 2] |= (uint32_t(surface_index(zS, zT, Ls)) & 0xffu)
                     << (8 * (t & 3));
     }
   }
