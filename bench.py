@@ -48,6 +48,8 @@ def parse():
   p.add_argument("--no-graph", action="store_true")
   p.add_argument("--streams", type=int, default=4,
                  help="concurrent job groups in the fused step")
+  p.add_argument("--deterministic", action="store_true",
+                 help="fixed-point (order-independent) gradient sums")
   p.add_argument("--fused-transform", action="store_true",
                  help="transform vertices inside the raster and splat loads "
                  "instead of a transform launch into clip-space buffers")
@@ -416,7 +418,8 @@ def main():
 
   r = batch.BatchedSplatRenderer([w["mesh"]], jobs, L, tile="auto",
                                  streams=args.streams,
-                                 fused_transform=args.fused_transform)
+                                 fused_transform=args.fused_transform,
+                                 deterministic=args.deterministic)
   log(f"autotuned raster tile {r.raster_config.tile_w}x"
       f"{r.raster_config.tile_h}, bin capacity {r.raster_config.bin_capacity}"
       f", timings {r.autotune_timings}")
@@ -548,6 +551,8 @@ def main():
                              f"fused path on {args.streams} stream(s)" +
                              (", transform on load" if args.fused_transform
                               else "") +
+                             (", deterministic gradients" if
+                              args.deterministic else "") +
                              (" in a HIP graph" if use_graph else ""),
                  "triangles": T, "vertices": V, "resolution": R, "layers": L,
                  "views": B, "channels": C, "parallelism": "single GPU"},

@@ -63,7 +63,8 @@ class SplatJob(ctypes.Structure):
               ("accum_weights", _c_ptr),
               ("num_vertices", _c_int),
               ("num_triangles", _c_int), ("width", _c_int),
-              ("height", _c_int), ("tile_begin", _c_i64)]
+              ("height", _c_int), ("tile_begin", _c_i64),
+              ("fixed_shift", _c_int), ("reserved", _c_int)]
 
 
 # name -> (restype, argtypes).  Keep in the order of include/diffren_amd.h.
@@ -146,6 +147,7 @@ SIGNATURES = {
     "dr_render_splat_forward": (_c_int, [_c_ptr, _c_int, _c_i64, _c_int,
                                          _c_int, _c_int, ctypes.c_float,
                                          _c_ptr, _c_ptr, _c_size, _c_ptr]),
+    "dr_fixed_to_float": (_c_int, [_c_ptr, _c_i64, _c_int, _c_ptr, _c_ptr]),
     "dr_render_splat_backward": (_c_int, [_c_ptr, _c_int, _c_i64, _c_int,
                                           _c_int, _c_ptr]),
 }

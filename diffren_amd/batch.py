@@ -60,6 +60,13 @@ class BatchedSplatRenderer:
            209-212; rasterization held fixed as rasterize_triangles_xla.py:
            51-52 holds it).
 
+  deterministic: accumulate the vertex (and camera) gradients as exact
+           64-bit fixed-point sums with resolution 2^-fixed_shift (range
+           |g| < 2^(63 - fixed_shift)), converted to float32 after the
+           backward: the gradients then do not depend on the order in which
+           the GPU's atomics arrive, so repeated steps give identical bits.
+           The loss is always summed in a fixed order.
+
   After step(): images[i], loss (sum over jobs of loss_scale * |img - tgt|^2),
   d_vertices[m] [V,3] and d_colors[m] [V,C] (views of records[m]).  ids[i]
   holds triangle ids; z[i] / barys[i] are defined under triangles only.
@@ -68,7 +75,7 @@ class BatchedSplatRenderer:
   def __init__(self, meshes, jobs, num_layers, tile=None, bin_capacity=0,
                loss_scale=None, autotune_cache=None, streams=1,
                fused_transform=False, return_accum_buffers=False,
-               camera_grads=False):
+               camera_grads=False, deterministic=False, fixed_shift=40):
     lib = _native.lib()
     self.meshes = meshes
     self.jobs = jobs
@@ -111,6 +118,11 @@ class BatchedSplatRenderer:
                     for m in meshes]
     self.d_vertices = [r[:, 0:3] for r in self.records]
     self.d_colors = [r[:, 4:4 + C] for r in self.records]
+    self.fixed_shift = int(fixed_shift) if deterministic else 0
+    if deterministic and not 1 <= self.fixed_shift <= 62:
+      raise ValueError("fixed_shift must be in [1, 62]")
+    self.records_fixed = ([torch.zeros(r.shape, dtype=torch.int64, device=dev)
+                           for r in self.records] if deterministic else [])
     self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
     self.ids, self.z, self.barys = [], [], []
     self.images, self.targets, self.scratch = [], [], []
@@ -133,6 +145,9 @@ class BatchedSplatRenderer:
     sf = lib.dr_render_scratch_floats(C)
     self.d_cameras = (torch.zeros((len(jobs), 4, 4), dtype=torch.float32,
                                   device=dev) if camera_grads else None)
+    self.d_cameras_fixed = (torch.zeros((len(jobs), 4, 4), dtype=torch.int64,
+                                        device=dev)
+                            if camera_grads and deterministic else None)
     self.accum, self.norm, self.accum_weights = [], [], []
     if return_accum_buffers:
       for _, _, w, h in jobs:
@@ -162,6 +177,11 @@ class BatchedSplatRenderer:
     self.autotune_cache = autotune_cache
     self.autotune_timings = None
     self.groups = self._split(max(1, int(streams)))
+    # One loss slot per group, summed in a fixed order after the groups join.
+    self.loss_parts = torch.zeros(len(self.groups), dtype=torch.float32,
+                                  device=dev)
+    for k, g in enumerate(self.groups):
+      g.loss_slot = self.loss_parts[k:k + 1]
     self.streams = [torch.cuda.Stream(device=dev) for _ in self.groups] \
         if len(self.groups) > 1 else []
     if tile == "auto":
@@ -263,7 +283,8 @@ class BatchedSplatRenderer:
       if not self.fused_transform:
         j.clip_vertices = self.clip[i].data_ptr()
       if self.d_cameras is not None:
-        j.d_camera = self.d_cameras[i].data_ptr()
+        j.d_camera = (self.d_cameras_fixed if self.d_cameras_fixed is not None
+                      else self.d_cameras)[i].data_ptr()
       j.triangles = m["triangles"].data_ptr()
       j.colors = m["colors"].data_ptr()
       j.camera = self.cameras[i].data_ptr()
@@ -274,7 +295,9 @@ class BatchedSplatRenderer:
       j.target = self.targets[i].data_ptr()
       j.d_image = None
       j.grad_scratch = self.scratch[i].data_ptr()
-      j.d_records = self.records[mi].data_ptr()
+      j.d_records = (self.records_fixed[mi] if self.records_fixed else
+                     self.records[mi]).data_ptr()
+      j.fixed_shift = self.fixed_shift
       if self.accum:
         j.accum = self.accum[i].data_ptr()
         j.norm = self.norm[i].data_ptr()
@@ -327,7 +350,7 @@ class BatchedSplatRenderer:
             _native.ptr(g.splat_table), len(g.indices), g.splat_tiles, self.L,
             self.C,
             grad_mode | (_native.RENDER_ACCUM_BUFFERS if self.accum else 0),
-            self.loss_scale, _native.ptr(self.loss),
+            self.loss_scale, _native.ptr(g.loss_slot),
             _native.ptr(g.splat_ws), g.splat_ws.numel(),
             _native.stream_handle()), "dr_render_splat_forward")
 
@@ -344,17 +367,40 @@ class BatchedSplatRenderer:
   def splat_forward(self, grad_mode=1):
     for g in self.groups:
       self._splat_forward(g, grad_mode)
+    self._sum_loss()
 
   def splat_backward(self):
     for g in self.groups:
       self._splat_backward(g)
+    self._convert_fixed()
+
+  def _sum_loss(self):
+    # Fixed order: elementwise adds, no reduction kernel.
+    self.loss.copy_(self.loss_parts[0:1])
+    for k in range(1, len(self.groups)):
+      self.loss.add_(self.loss_parts[k:k + 1])
+
+  def _convert_fixed(self):
+    """Deterministic mode: integer gradient sums -> float32 records."""
+    if not self.fixed_shift:
+      return
+    lib = _native.lib()
+    pairs = list(zip(self.records_fixed, self.records))
+    if self.d_cameras_fixed is not None:
+      pairs.append((self.d_cameras_fixed, self.d_cameras))
+    for src, dst in pairs:
+      _native.check(lib.dr_fixed_to_float(
+          _native.ptr(src), src.numel(), self.fixed_shift, _native.ptr(dst),
+          _native.stream_handle()), "dr_fixed_to_float")
 
   def zero_grads(self):
     # Fill kernels, not memsets, so the step can be graph-captured.
-    for t in self.records:
+    for t in self.records + self.records_fixed:
       t.zero_()
-    if self.d_cameras is not None:
-      self.d_cameras.zero_()
+    for t in (self.d_cameras, self.d_cameras_fixed):
+      if t is not None:
+        t.zero_()
+    self.loss_parts.zero_()
     self.loss.zero_()
 
   def _group_step(self, g, grad_mode):
@@ -368,15 +414,19 @@ class BatchedSplatRenderer:
   def _run_groups(self, grad_mode):
     if not self.streams:
       self._group_step(self.groups[0], grad_mode)
-      return
-    cur = torch.cuda.current_stream()
-    for s in self.streams:
-      s.wait_stream(cur)
-    for g, s in zip(self.groups, self.streams):
-      with torch.cuda.stream(s):
-        self._group_step(g, grad_mode)
-    for s in self.streams:
-      cur.wait_stream(s)
+    else:
+      cur = torch.cuda.current_stream()
+      for s in self.streams:
+        s.wait_stream(cur)
+      for g, s in zip(self.groups, self.streams):
+        with torch.cuda.stream(s):
+          self._group_step(g, grad_mode)
+      for s in self.streams:
+        cur.wait_stream(s)
+    if grad_mode == 1:
+      self._sum_loss()
+    if grad_mode:
+      self._convert_fixed()
 
   def _check_overflow(self):
     """Regrows the bins if the last completed step overflowed them."""

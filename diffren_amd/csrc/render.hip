@@ -564,15 +564,33 @@ __device__ __forceinline__ float record_entry(float b, const float* base,
   return c < 4 + C ? b * base[c] : 0.0f;
 }
 
+// Adds v to a gradient entry: a float atomic, or -- in the deterministic
+// mode (fixed_shift > 0) -- an exact 64-bit integer atomic of v * 2^shift
+// rounded to the nearest integer, so the sum does not depend on the order
+// in which the additions arrive (the entry's memory holds an int64).
+__device__ __forceinline__ void grad_add(float* p, int64_t i, float v,
+                                         int fixed_shift, float fixed_scale) {
+  if (fixed_shift > 0) {
+    const long long q = llrintf(v * fixed_scale);
+    atomicAdd(reinterpret_cast<unsigned long long*>(p) + i,
+              static_cast<unsigned long long>(q));
+  } else {
+    gatomic_add(p + i, v);
+  }
+}
+
 template <int C>
 __device__ __forceinline__ void flush_record(float* grads, int v, float b,
-                                             const float* base) {
+                                             const float* base,
+                                             int fixed_shift,
+                                             float fixed_scale) {
   constexpr int R = Record<C>::kSize;
   if (grads == nullptr) return;
 #pragma unroll
   for (int c = 0; c < R; ++c) {
     const float e = record_entry<C>(b, base, c);
-    if (e != 0.0f) gatomic_add(grads + int64_t(v) * R + c, e);
+    if (e != 0.0f)
+      grad_add(grads, int64_t(v) * R + c, e, fixed_shift, fixed_scale);
   }
 }
 
@@ -630,6 +648,8 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   if ((J.d_records == nullptr && J.d_camera == nullptr) ||
       J.grad_scratch == nullptr)
     return;
+  const int fshift = J.fixed_shift;
+  const float fscale = fshift > 0 ? ldexpf(1.0f, fshift) : 1.0f;
   const bool inside = x < W && y < H;
   const int64_t P = int64_t(W) * H;
   const int64_t p = int64_t(y) * W + x;
@@ -803,7 +823,8 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
           float d = dclip[i] * ph[j];
           for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off);
           if (lane == 0 && d != 0.0f)
-            gatomic_add(J.d_camera + (i < 2 ? 4 * i : 12) + j, d);
+            grad_add(J.d_camera, (i < 2 ? 4 * i : 12) + j, d, fshift,
+                     fscale);
         }
     }
     // Wave-local dedupe into compact LDS rows (one per distinct vertex, in
@@ -857,7 +878,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
           for (int c = 0; c < R; ++c)
             val[r * RS + c] = record_entry<C>(k.b[i], base, c);
         } else {
-          flush_record<C>(grads, k.v[i], k.b[i], base);
+          flush_record<C>(grads, k.v[i], k.b[i], base, fshift, fscale);
         }
       }
       nr += __popcll(fm);
@@ -870,7 +891,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         if (first[i]) continue;
         const int r = hrow[slot[i]];
         if (r < 0) {
-          flush_record<C>(grads, k.v[i], k.b[i], base);
+          flush_record<C>(grads, k.v[i], k.b[i], base, fshift, fscale);
           continue;
         }
 #pragma unroll
@@ -888,13 +909,21 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         const int e = e0 + lane / R, c = lane % R;
         if (e < n) {
           const float g = val[e * RS + c];
-          if (g != 0.0f) gatomic_add(grads + int64_t(vtx[e]) * R + c, g);
+          if (g != 0.0f)
+            grad_add(grads, int64_t(vtx[e]) * R + c, g, fshift, fscale);
         }
       }
     }
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     __builtin_amdgcn_wave_barrier();
   }
+}
+
+__global__ void fixed_to_float_kernel(const int64_t* __restrict__ fixed,
+                                      int64_t n, double scale,
+                                      float* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = float(double(fixed[i]) * scale);
 }
 
 size_t target_lds_bytes(int L, int C) {
@@ -1019,6 +1048,12 @@ int dr_splat_jobs_prepare(dr_splat_job* jobs, int32_t n, int32_t channels,
                  "grad_scratch must be 16-byte aligned");
     DR_CHECK_ARG(j.d_camera == nullptr || (j.camera && j.positions),
                  "d_camera needs camera and positions");
+    DR_CHECK_ARG(j.fixed_shift >= 0 && j.fixed_shift <= 62,
+                 "fixed_shift must be in [0, 62]");
+    DR_CHECK_ARG(j.fixed_shift == 0 ||
+                     ((reinterpret_cast<uintptr_t>(j.d_records) & 7) == 0 &&
+                      (reinterpret_cast<uintptr_t>(j.d_camera) & 7) == 0),
+                 "fixed-point gradients need 8-byte aligned buffers");
     j.tile_begin = tiles;
     tiles += int64_t((j.width + dr::kBX - 1) / dr::kBX) *
              ((j.height + dr::kBY - 1) / dr::kBY);
@@ -1066,6 +1101,21 @@ int dr_render_splat_forward(const dr_splat_job* jobs_device, int32_t num_jobs,
   DR_RENDER_SWITCH(channels, DR_CALL)
 #undef DR_CALL
   return dr::fail(dr::kUnsupported, "unsupported channel count");
+}
+
+int dr_fixed_to_float(const int64_t* fixed, int64_t n, int32_t fixed_shift,
+                      float* out, dr_stream_t stream) {
+  DR_CHECK_ARG(n >= 0, "negative count");
+  DR_CHECK_ARG(fixed_shift > 0 && fixed_shift <= 62,
+               "fixed_shift must be in [1, 62]");
+  if (n == 0) return DR_OK;
+  DR_CHECK_ARG(fixed && out, "null pointer");
+  const unsigned blocks = unsigned((n + 255) / 256);
+  dr::fixed_to_float_kernel<<<blocks, 256, 0,
+                              static_cast<hipStream_t>(stream)>>>(
+      fixed, n, ldexp(1.0, -fixed_shift), out);
+  DR_CHECK_LAUNCH("fixed to float");
+  return DR_OK;
 }
 
 int dr_render_splat_backward(const dr_splat_job* jobs_device,

@@ -357,6 +357,16 @@ typedef struct dr_splat_job {
   int32_t width;
   int32_t height;
   int64_t tile_begin;           /* filled by dr_splat_jobs_prepare */
+  int32_t fixed_shift;          /* 0: d_records and d_camera accumulate
+                                   float atomics (summation order not
+                                   fixed).  1-62: deterministic mode: both
+                                   hold int64 (8-byte aligned) and each
+                                   contribution adds round(v * 2^shift)
+                                   exactly, so the result does not depend
+                                   on the order; dr_fixed_to_float
+                                   converts.  Range |sum| < 2^(63-shift),
+                                   resolution 2^-shift. */
+  int32_t reserved;             /* 0 */
 } dr_splat_job;
 
 /* Host: fills tile_begin of jobs[0..n) and returns the total tile count. */
@@ -396,6 +406,11 @@ int dr_render_splat_forward(const dr_splat_job* jobs_device, int32_t num_jobs,
                             int32_t channels, int32_t grad_mode,
                             float loss_scale, float* loss, void* workspace,
                             size_t workspace_bytes, dr_stream_t stream);
+
+/* out[i] = fixed[i] * 2^-fixed_shift: the deterministic mode's integer
+ * gradient sums as float32.  No reference counterpart. */
+int dr_fixed_to_float(const int64_t* fixed, int64_t n, int32_t fixed_shift,
+                      float* out, dr_stream_t stream);
 
 /* Source pass: from grad_scratch (written by a forward with grad_mode 1 or
  * 2), accumulate the d_records of all jobs (float atomics); jobs with a NULL
