@@ -33,7 +33,9 @@ namespace dr {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kMaxLayers = 8;
+// Depth layers per pixel; the tile's L-deep key lists must also fit the
+// 64 KiB LDS budget (L x tile pixels x 8 bytes).
+constexpr int kMaxLayers = 64;
 constexpr uint64_t kEmpty = ~uint64_t(0);
 // Clipped-bbox area above which a triangle is rasterized by a whole wave
 // instead of a single lane.
@@ -847,7 +849,7 @@ __device__ __forceinline__ bool item_fragment(const dr_raster_job& job,
 __device__ __forceinline__ void replay_pixel(
     const dr_raster_job& job, const TileItems& it, int64_t items, int ix,
     int iy, int p, float sx2, float sy2, int cull, int L, uint64_t* slot,
-    int tp, uint64_t* cand, int* cand_n) {
+    int tp, uint64_t* cand, int* cand_n, float* replay_z, int* replay_id) {
   const uint64_t last = slot[(L - 1) * tp];
   const uint32_t zmax = last == kEmpty ? 0xffffffffu : uint32_t(last >> 32);
   if (threadIdx.x == 0) *cand_n = 0;
@@ -864,8 +866,8 @@ __device__ __forceinline__ void replay_pixel(
   __syncthreads();
   if (threadIdx.x == 0) {
     const int n = *cand_n;
-    float lz[kMaxLayers];
-    int lid[kMaxLayers];
+    float* lz = replay_z;
+    int* lid = replay_id;
     for (int l = 0; l < L; ++l) {
       lz[l] = 1.0f;
       lid[l] = -1;
@@ -938,6 +940,9 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
   __shared__ uint32_t ties[(TP + 31) / 32];
   __shared__ uint64_t cand[kMaxCandidates];
   __shared__ int cand_n;
+  // The replaying thread's L-deep list.
+  __shared__ float replay_z[kMaxLayers];
+  __shared__ int replay_id[kMaxLayers];
 
   const int64_t flat_tile = xcd_block<4>();
   // One job per block: its fields are held in scalar registers (loaded
@@ -1093,7 +1098,7 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
       const int ix = x0 + p % TW, iy = y0 + p / TW;
       if (ix >= x1 || iy >= y1) continue;
       replay_pixel(job, it, items, ix, iy, p, sx2, sy2, cull, L, keys + p, TP,
-                   cand, &cand_n);
+                   cand, &cand_n, replay_z, replay_id);
     }
   }
 
@@ -1226,6 +1231,13 @@ int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
   }
   if (!run_tiles) return kOk;
   const size_t lds = sizeof(uint64_t) * L * TW * TH;
+  if (lds > 32768) {
+    // Deep key lists: dynamic LDS plus the kernel's static arrays may pass
+    // the default 64 KiB launch limit.
+    DR_CHECK_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&raster_tile_kernel<TW, TH>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  }
   raster_tile_kernel<TW, TH><<<unsigned(total_tiles), kBlock, lds, s>>>(
       J, L, cull, w.counts, w.offsets, w.bin_key, w.bin_pix, w.head,
       reinterpret_cast<const uint32_t*>(w.frag_z), capacity,
@@ -1240,7 +1252,7 @@ int dispatch(const dr_raster_job* table, const dr_raster_job* single,
              int count, int64_t total_tris, int64_t total_tiles, int L,
              int cull, const dr_raster_config& c, void* ws, size_t ws_bytes,
              hipStream_t s) {
-  DR_CHECK_ARG(L >= 1 && L <= kMaxLayers, "num_layers must be in [1, 8]");
+  DR_CHECK_ARG(L >= 1 && L <= kMaxLayers, "num_layers must be in [1, 64]");
   DR_CHECK_ARG(cull >= 0 && cull <= 2, "face_culling_mode must be 0, 1 or 2");
   DR_CHECK_ARG((c.flags & DR_RASTER_BINS_ONLY) == 0 ||
                    (c.flags & DR_RASTER_TILES_ONLY) == 0,

@@ -32,11 +32,15 @@ from diffren_amd import framebuffer as fb
 from diffren_amd import transforms
 
 DEFAULT_TILE = (32, 8)
-MAX_LAYERS = 8
+# The rasterizer keeps num_layers x tile pixels 64-bit depth keys per tile in
+# 64 KiB of LDS: up to 32 layers on the default 32x8 tiles, 64 on 8x8 tiles.
+MAX_LAYERS = 64
 
 
-def _config(tile=None, bin_capacity=0, flags=0):
+def _config(tile=None, bin_capacity=0, flags=0, num_layers=1):
   tw, th = tile or DEFAULT_TILE
+  if tile is None and num_layers * tw * th * 8 > 65536:
+    tw, th = 8, 8
   return _native.RasterConfig(int(tw), int(th), int(bin_capacity),
                               int(flags), 0)
 
@@ -91,7 +95,7 @@ def rasterize_batched(vertices_list, triangles_list, sizes, num_layers,
   vertices_list[i] [Vi,4] f32 and triangles_list[i] [Ti,3] i32 are device
   tensors; sizes[i] = (width, height).  Returns a list of (ids, z, barys).
   """
-  config = _config(tile, bin_capacity)
+  config = _config(tile, bin_capacity, num_layers=num_layers)
   n = len(vertices_list)
   dev = vertices_list[0].device
   jobs = (_native.RasterJob * n)()
@@ -146,7 +150,7 @@ def rasterize_triangles_primitive(vertices, triangles, image_width,
   _native.require_device(vertices, triangles)
   vertices = vertices.detach().to(torch.float32).contiguous()
   triangles = triangles.to(torch.int32).contiguous()
-  config = _config(tile, bin_capacity)
+  config = _config(tile, bin_capacity, num_layers=num_layers)
   if vertices.dim() == 2 and triangles.dim() == 2:
     return _rasterize_one(vertices, triangles, image_width, image_height,
                           num_layers, face_culling_mode, config)

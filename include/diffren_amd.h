@@ -121,7 +121,8 @@ size_t dr_raster_workspace_size(int64_t total_triangles, int64_t total_tiles,
  * Triangle ids, z and barycentrics are bit-identical to the reference CPU
  * kernel diffren::RasterizeTrianglesImpl (rasterize_triangles_impl.cc:27-118).
  * Triangles with a vertex index outside [0, num_vertices) are skipped.
- * num_layers must be in [1, 8]. */
+ * num_layers must be in [1, 64], with num_layers x tile_w x tile_h x 8
+ * bytes of depth keys at most 64 KiB (8x8 tiles take up to 64 layers). */
 int dr_rasterize_triangles(const float* vertices, int32_t num_vertices,
                            const int32_t* triangles, int32_t num_triangles,
                            int32_t width, int32_t height, int32_t num_layers,

@@ -159,7 +159,7 @@ def test_invalid_arguments_raise(device):
   with pytest.raises(ValueError):
     rasterize.rasterize_triangles_primitive(vt, tt, 0, 4, 1, 0)
   with pytest.raises(ValueError):
-    rasterize.rasterize_triangles_primitive(vt, tt, 4, 4, 9, 0)
+    rasterize.rasterize_triangles_primitive(vt, tt, 4, 4, 65, 0)
   with pytest.raises(ValueError):
     rasterize.rasterize_triangles_primitive(vt, tt, 4, 4, 1, 3)
 
@@ -180,7 +180,7 @@ def _stack(num, rng, coplanar):
   return np.array(v, np.float32), t
 
 
-@pytest.mark.parametrize("layers", [1, 2, 4, 8])
+@pytest.mark.parametrize("layers", [1, 2, 4, 8, 16, 40])
 def test_exact_depth_ties_follow_reference_order(layers, device):
   # Every fragment has the same z: the kept ids and their order are those of
   # the reference's id-ordered insertion with strict '<', not simply the
@@ -198,6 +198,17 @@ def test_eight_layer_stack(device):
   got = gpu_raster(v, t, 64, 64, 8, 0, device)
   assert_same(got, raster_oracle.rasterize(v, t, 64, 64, 8, 0))
   assert (got[0][7] >= 0).any()  # the deepest layer is populated
+
+
+@pytest.mark.parametrize("layers,tile", [(24, None), (64, None), (32, (16, 16))])
+def test_deep_layer_stacks(layers, tile, device):
+  # More layers than a splat uses: 24 and 32 fit the default and 16x16
+  # tiles' key lists, 64 switches to 8x8 tiles (the LDS key budget).
+  rng = np.random.default_rng(layers)
+  v, t = _stack(90, rng, coplanar=False)
+  got = gpu_raster(v, t, 64, 56, layers, 0, device, tile=tile)
+  assert_same(got, raster_oracle.rasterize(v, t, 64, 56, layers, 0))
+  assert (got[0][min(layers, 40) - 1] >= 0).any()
 
 
 @pytest.mark.parametrize("w,h", [(1, 1), (2048, 160), (160, 2048), (31, 1)])
