@@ -170,6 +170,38 @@ def test_render_splat_forward_and_gradients(layers, device):
   assert helpers.rel_err(v.grad.cpu(), v64.grad) <= GRAD_TOL
 
 
+def test_recompute_rasterization(device):
+  """render_triangles(recompute_rasterization=True): the backward
+  rasterizes again instead of keeping the framebuffer; image and gradients
+  are the stored path's, and less memory stays allocated between forward
+  and backward."""
+  w, h, L = 512, 384, 3
+  s = scenes.icosphere_scene(seed=6, subdivisions=4)
+  out = []
+  for recompute in (False, True):
+    v = torch.as_tensor(s["vertices"], device=device).requires_grad_(True)
+    col = torch.as_tensor(s["colors"], device=device).requires_grad_(True)
+    cam = torch.as_tensor(s["camera"], device=device)
+    tris = torch.as_tensor(s["triangles"], device=device)
+    torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated()
+    img = render.render_triangles(v, {"c": col}, tris, cam, w, h,
+                                  lambda a: a["c"], num_layers=L,
+                                  recompute_rasterization=recompute)
+    held = torch.cuda.memory_allocated() - base
+    loss = (img * img).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    out.append((img.detach(), v.grad, col.grad, held))
+  (i0, v0, c0, m0), (i1, v1, c1, m1) = out
+  print(f"memory held between forward and backward: stored {m0 / 2**20:.1f} "
+        f"MiB, recomputed {m1 / 2**20:.1f} MiB")
+  assert torch.equal(i0, i1)
+  assert helpers.rel_err(v1.cpu(), v0.cpu()) <= 1e-5
+  assert helpers.rel_err(c1.cpu(), c0.cpu()) <= 1e-5
+  assert m1 < m0 / 4
+
+
 @pytest.mark.parametrize("num_components", [4, 2])
 def test_optimize_single_triangle(num_components, device):
   """splat_test.py:84-168: gradient descent on a triangle's clip-space
