@@ -481,7 +481,7 @@ def main():
   kernel_names = {
       "raster_tiles": "raster_tile_kernel<%d, %d>" % (
           r.raster_config.tile_w, r.raster_config.tile_h),
-      "splat_target": "target_kernel<4>",
+      "splat_target": "target_kernel<4, false>",
       "splat_source": "source_kernel<4>",
   }
   launch_ms = {k: time_launches(r, k, reps)
