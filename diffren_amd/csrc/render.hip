@@ -71,6 +71,14 @@ __device__ __forceinline__ float* scratch_class(float* base, int64_t P,
                 : base + P * S::kHot + p * S::kCold + (k == 0 ? 0 : S::kStride);
 }
 
+// Orders a wave's LDS writes before its lanes' later LDS reads (release /
+// acquire at wavefront scope around the wave barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ int find_job(const dr_splat_job* jobs, int n,
                                         int64_t tile) {
   int lo = 0, hi = n - 1;
@@ -698,8 +706,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
       }
     }
   }
-  __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-  __builtin_amdgcn_wave_barrier();
+  wave_sync();
 
   // Surface indices of this source with its 9 targets, as the target pass
   // computed them: byte t of each target's record.  One signed byte per
@@ -832,8 +839,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     const uint64_t act = __ballot(frag);
     if (act == 0) continue;
     for (int i = lane; i < kHashSlots; i += 64) hkey[i] = -1;
-    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();
     float* grads = J.d_records;
     int slot[3] = {0, 0, 0};
     bool first[3] = {false, false, false};
@@ -883,8 +889,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
       }
       nr += __popcll(fm);
     }
-    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();
     if (frag) {
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
@@ -901,8 +906,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         }
       }
     }
-    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();
     const int n = min(nr, kRows);
     if (grads) {
       for (int e0 = 0; e0 < n; e0 += 64 / R) {
@@ -914,8 +918,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         }
       }
     }
-    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();
   }
 }
 
