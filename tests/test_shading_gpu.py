@@ -101,6 +101,26 @@ def test_mlp_gradient_matches_oracle(modulate, device):
     assert np.linalg.norm(got[a:b] - g[a:b]) <= 2e-2 * np.linalg.norm(g[a:b])
 
 
+@pytest.mark.parametrize("n", [1, 40, 70001])
+def test_mlp_gradient_sizes(n, device):
+  """Default grid: waves with no chunk (n = 1, 40), ragged last chunk, and
+  every block busy over several chunks (70001)."""
+  rng = np.random.default_rng(n)
+  p = mo.init_params(6)
+  x = rng.normal(size=(n, 12)).astype(np.float32)
+  t = rng.normal(size=(n, 3)).astype(np.float32)
+  scale = 1.0 / (3 * n)
+  net = shading.MLPShader(p, device)
+  net.loss_and_grad(torch.as_tensor(x, device=device),
+                    torch.as_tensor(t, device=device), None, scale)
+  loss, g = mo.loss_and_grad(p, x, t, np.ones(n, np.float32), scale)
+  got = net.grad.cpu().numpy()
+  assert abs(net.loss.item() - loss) <= 1e-3 * abs(loss)
+  for (a, b), _, (c, d) in mo.param_slices()[0]:
+    assert np.linalg.norm(got[a:b] - g[a:b]) <= 2e-2 * np.linalg.norm(g[a:b])
+    assert np.linalg.norm(got[c:d] - g[c:d]) <= 2e-2 * np.linalg.norm(g[c:d])
+
+
 def test_mlp_adam_matches_oracle(device):
   p = mo.init_params(5)
   rng = np.random.default_rng(2)
