@@ -105,6 +105,7 @@ __device__ __forceinline__ bf16x8 zero8() {
 }
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint32_t cvt2(float a, float b) {
@@ -237,8 +238,9 @@ __device__ __forceinline__ Fetch fetch(const float* __restrict__ x,
   f.t0 = ldg(target + q * 3);
   f.t1 = ldg(target + q * 3 + 1);
   f.t2 = ldg(target + q * 3 + 2);
-  const float m = ldg((mask ? mask : target) + q);
-  f.mk = mask ? m : 1.0f;
+  // The raw value (the target when there is no mask): selecting 1 here
+  // would make the compiler wait for this load as soon as it is issued.
+  f.mk = ldg((mask ? mask : target) + q);
   return f;
 }
 
@@ -585,11 +587,12 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
                             modulate ? cur.xc.z : 1.0f,
                             modulate ? cur.xc.w : 1.0f};
       const float t[3] = {cur.t0, cur.t1, cur.t2};
+      const float mk = mask ? cur.mk : 1.0f;
 #pragma unroll
       for (int r = 0; r < 3; ++r) {
         const float e = alb[r] * y[r] - t[r];
-        partial += cur.mk * (e * e) * scale;
-        d5[r] = (((2.0f * e) * scale) * cur.mk) * alb[r];
+        partial += mk * (e * e) * scale;
+        d5[r] = (((2.0f * e) * scale) * mk) * alb[r];
       }
     }
     bf16x8 dd[4];
@@ -680,7 +683,10 @@ __global__ __launch_bounds__(kBlock) void train_kernel(
         v.z += o.z;
         v.w += o.w;
       }
-      out4[half * kHalfTiles * 256 + i] = v;
+      // Streamed past the caches: the row is read back by another kernel.
+      const f32x4 w = {v.x, v.y, v.z, v.w};
+      __builtin_nontemporal_store(
+          w, reinterpret_cast<f32x4*>(out4 + half * kHalfTiles * 256 + i));
     }
     __syncthreads();
   }
