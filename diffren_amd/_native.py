@@ -23,6 +23,7 @@ RASTER_BINS_ONLY = 2
 RASTER_TILES_ONLY = 4
 RENDER_ACCUM_BUFFERS = 4
 MLP_ALBEDO_MODULATE = 1
+MLP_GRAD_OVERWRITE = 2
 PAD_CONSTANT = 0
 PAD_EDGE = 1
 

@@ -75,8 +75,6 @@ constexpr int kRawBias = kTiles * 1024;
 constexpr int kRawLoss = kRawBias + 4 * 2 * 64;
 constexpr int kRawUsed = kRawLoss + 64;
 constexpr int kRawStride = kRawUsed;      // 16-byte rows
-constexpr int kStages = 16;               // rows of the first sum pass
-constexpr int kSumBlocks = (kRawUsed + 255) / 256;
 static_assert(kRawStride % 4 == 0 && kRawLoss % 256 == 0 &&
                   kRawUsed == kRawLoss + 64,
               "sizes");
@@ -728,53 +726,57 @@ __device__ int tile_param(int e) {
   return kW[layer] + unit_of(ip) * kHid + unit_of(op);
 }
 
-__device__ __forceinline__ float stage_sum(const float* stage, int e) {
-  float t = 0.0f;
-#pragma unroll
-  for (int g = 0; g < kStages; ++g) t += stage[int64_t(g) * kRawStride + e];
-  return t;
-}
 
-// stage[g][e] = sum of block rows g, g + kStages, ... (fixed order).
-__global__ __launch_bounds__(256) void sum_rows_kernel(
-    const float* __restrict__ partials, int nblocks, float* __restrict__ stage) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= kRawUsed) return;
+// Sum of the block rows of 64 raw columns per workgroup, one pass: 16
+// thread groups each add every 16th row, then group 0 adds the 16 group sums
+// in a fixed order and scatters the result to the parameters (a bias
+// position's two lane halves, and the 64 per-lane losses, fall in the same
+// 64 columns).  overwrite: grad = sum (else grad += sum), and loss likewise.
+constexpr int kSumCols = 64, kSumGroups = 16;
+static_assert(kRawBias % kSumCols == 0 && kRawLoss % kSumCols == 0 &&
+                  kRawUsed % kSumCols == 0,
+              "raw sections align with the column blocks");
+
+__global__ __launch_bounds__(kSumCols * kSumGroups) void sum_kernel(
+    const float* __restrict__ partials, int nblocks, float* __restrict__ grad,
+    float* __restrict__ loss, int overwrite) {
+  __shared__ float red[kSumGroups][kSumCols];
+  const int c = threadIdx.x % kSumCols, grp = threadIdx.x / kSumCols;
+  const int e = blockIdx.x * kSumCols + c;
   float t = 0.0f;
 #pragma unroll 8
-  for (int b = blockIdx.y; b < nblocks; b += kStages)
+  for (int b = grp; b < nblocks; b += kSumGroups)
     t += partials[int64_t(b) * kRawStride + e];
-  stage[int64_t(blockIdx.y) * kRawStride + e] = t;
-}
-
-// grad[param] += the stage rows' sum of its raw entry (both lane halves for
-// a bias); *loss += the sum of the per-lane losses.
-__global__ __launch_bounds__(256) void finish_kernel(
-    const float* __restrict__ stage, float* __restrict__ grad,
-    float* __restrict__ loss) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= kRawUsed) return;
+  red[grp][c] = t;
+  __syncthreads();
+  if (grp != 0) return;
+  t = red[0][c];
+#pragma unroll
+  for (int g = 1; g < kSumGroups; ++g) t += red[g][c];
+  auto put = [&](float* dst, float v) { *dst = overwrite ? v : *dst + v; };
   if (e < kRawBias) {
     const int i = tile_param(e);
-    if (i >= 0) grad[i] += stage_sum(stage, e);
+    if (i >= 0) put(grad + i, t);
   } else if (e < kRawLoss) {
     // Bias sums: lanes l and l + 32 hold the same position.
-    const int b = e - kRawBias, lane = b & 63;
-    if (lane >= 32) return;
-    const int layer = 1 + (b >> 7), pos = 32 * ((b >> 6) & 1) + lane;
-    const float t = stage_sum(stage, e) + stage_sum(stage, e + 32);
+    red[0][c] = t;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (c >= 32) return;
+    const int b = e - kRawBias;
+    const int layer = 1 + (b >> 7), pos = 32 * ((b >> 6) & 1) + c;
+    const float v = red[0][c] + red[0][c + 32];
     if (layer == 4) {
-      if (pos < kOut) grad[kB[4] + pos] += t;
+      if (pos < kOut) put(grad + kB[4] + pos, v);
     } else {
-      grad[kB[layer] + unit_of(pos)] += t;
+      put(grad + kB[layer] + unit_of(pos), v);
     }
   } else {
-    // The 64 per-lane losses are the 64 lanes of one wave (kRawLoss is a
-    // multiple of 256).
-    float t = stage_sum(stage, e);
+    // The 64 per-lane losses.
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
-    if (e == kRawLoss && loss) *loss += t;
+    if (c == 0 && loss) put(loss, t);
   }
 }
 
@@ -786,7 +788,7 @@ extern "C" {
 
 size_t dr_mlp_train_workspace_size(int32_t num_blocks) {
   const int nb = num_blocks > 0 ? num_blocks : 256;
-  return (size_t(nb) + dr::mlp::kStages) * dr::mlp::kRawStride * sizeof(float);
+  return size_t(nb) * dr::mlp::kRawStride * sizeof(float);
 }
 
 int dr_mlp_train_step(const void* packed, const float* params, const float* x,
@@ -797,9 +799,19 @@ int dr_mlp_train_step(const void* packed, const float* params, const float* x,
                       size_t workspace_bytes, dr_stream_t stream) {
   namespace M = dr::mlp;
   DR_CHECK_ARG(num_pixels >= 0, "negative size");
-  if (num_pixels == 0) return DR_OK;
+  DR_CHECK_ARG(grad != nullptr, "null pointer");
+  const int overwrite = (flags & DR_MLP_GRAD_OVERWRITE) ? 1 : 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (num_pixels == 0) {
+    if (overwrite) {  // a sum over no block rows: zeros
+      M::sum_kernel<<<M::kRawUsed / M::kSumCols, M::kSumCols * M::kSumGroups,
+                      0, s>>>(nullptr, 0, grad, loss, 1);
+      DR_CHECK_LAUNCH("mlp train sum");
+    }
+    return DR_OK;
+  }
   DR_CHECK_ARG(num_pixels <= (int64_t(1) << 35), "more than 2^35 pixels");
-  DR_CHECK_ARG(packed && params && x && target && grad, "null pointer");
+  DR_CHECK_ARG(packed && params && x && target, "null pointer");
   DR_CHECK_ARG(workspace != nullptr, "workspace is null");
   DR_CHECK_ARG((reinterpret_cast<uintptr_t>(x) & 15) == 0,
                "features must be 16-byte aligned");
@@ -818,17 +830,13 @@ int dr_mlp_train_step(const void* packed, const float* params, const float* x,
   const int64_t chunks = (num_pixels + 31) / 32;
   const int nb = int(std::min<int64_t>(nb0, (chunks + M::kWaves - 1) / M::kWaves));
   float* partials = static_cast<float*>(workspace);
-  float* stage = partials + int64_t(nb) * M::kRawStride;
-  hipStream_t s = static_cast<hipStream_t>(stream);
   M::train_kernel<<<unsigned(nb), M::kBlock, M::kLdsBytes, s>>>(
       static_cast<const M::bf16x8*>(packed), x, target, mask, num_pixels,
       loss_scale, partials, input_norm, flags);
   DR_CHECK_LAUNCH("mlp train step");
-  M::sum_rows_kernel<<<dim3(M::kSumBlocks, M::kStages), 256, 0, s>>>(
-      partials, nb, stage);
+  M::sum_kernel<<<M::kRawUsed / M::kSumCols, M::kSumCols * M::kSumGroups, 0,
+                  s>>>(partials, nb, grad, loss, overwrite);
   DR_CHECK_LAUNCH("mlp train sum");
-  M::finish_kernel<<<M::kSumBlocks, 256, 0, s>>>(stage, grad, loss);
-  DR_CHECK_LAUNCH("mlp train finish");
   return DR_OK;
 }
 

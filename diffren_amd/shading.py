@@ -231,13 +231,12 @@ class MLPShader:
     if scale is None:
       cnt = n if mask is None else max(float(mk.sum().item()), 1.0)
       scale = 1.0 / (3.0 * cnt)
-    self.grad.zero_()
-    self.loss.zero_()
     _native.check(
         _native.lib().dr_mlp_train_step(
             _native.ptr(self.packed), _native.ptr(self.params),
             _native.ptr(x), _native.ptr(t), _native.ptr(mk), n, float(scale),
-            _native.ptr(self.norm), self.flags, _native.ptr(self.grad),
+            _native.ptr(self.norm), self.flags | _native.MLP_GRAD_OVERWRITE,
+            _native.ptr(self.grad),
             _native.ptr(self.loss), self.train_blocks,
             _native.ptr(self.train_ws), self.train_ws.numel(),
             _native.stream_handle()), "dr_mlp_train_step")

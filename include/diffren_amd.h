@@ -501,8 +501,10 @@ int dr_mlp_pack(const float* params, void* packed, dr_stream_t stream);
 /* Input normalisation and output options of the MLP entry points:
  * input_norm (device, may be NULL = identity) holds [shift 12 | scale 12]
  * and the network sees (x - shift) * scale; with DR_MLP_ALBEDO_MODULATE the
- * image is albedo * network output, albedo being x[:, 9:12] (raw). */
-enum { DR_MLP_ALBEDO_MODULATE = 1 };
+ * image is albedo * network output, albedo being x[:, 9:12] (raw).  With
+ * DR_MLP_GRAD_OVERWRITE dr_mlp_train_step writes grad and loss instead of
+ * adding to them. */
+enum { DR_MLP_ALBEDO_MODULATE = 1, DR_MLP_GRAD_OVERWRITE = 2 };
 
 /* y [num_pixels, 3] = MLP(x [num_pixels, 12]).  num_blocks <= 0: default. */
 int dr_mlp_forward(const void* packed, const float* params, const float* x,
@@ -514,8 +516,10 @@ size_t dr_mlp_train_workspace_size(int32_t num_blocks);
 
 /* One training pass: loss += loss_scale * sum_p mask[p] |y_p - target_p|^2
  * and grad += d loss / d params (float32, accumulated: the caller clears
- * grad and loss; per-block partials are summed in a fixed order, so the
- * result is deterministic).  mask [num_pixels] may be NULL (all ones). */
+ * grad and loss, or passes DR_MLP_GRAD_OVERWRITE; per-block partials are
+ * summed in a fixed order, so the result is deterministic).  mask
+ * [num_pixels] may be NULL (all ones); x must be 16-byte aligned and
+ * num_pixels at most 2^35. */
 int dr_mlp_train_step(const void* packed, const float* params, const float* x,
                       const float* target, const float* mask,
                       int64_t num_pixels, float loss_scale,

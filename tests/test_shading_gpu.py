@@ -121,6 +121,38 @@ def test_mlp_gradient_sizes(n, device):
     assert np.linalg.norm(got[c:d] - g[c:d]) <= 2e-2 * np.linalg.norm(g[c:d])
 
 
+def test_mlp_train_step_accumulates_or_overwrites(device):
+  """dr_mlp_train_step adds to grad/loss unless DR_MLP_GRAD_OVERWRITE; an
+  empty batch under overwrite leaves zeros."""
+  from diffren_amd import _native
+  rng = np.random.default_rng(3)
+  n = 3000
+  x = torch.as_tensor(rng.normal(size=(n, 12)).astype(np.float32),
+                      device=device)
+  t = torch.as_tensor(rng.normal(size=(n, 3)).astype(np.float32),
+                      device=device)
+  net = shading.MLPShader(mo.init_params(7), device)
+  net.loss_and_grad(x, t, None, 1.0 / n)
+  g1, l1 = net.grad.clone(), net.loss.clone()
+  lib = _native.lib()
+
+  def step(num, flags):
+    _native.check(lib.dr_mlp_train_step(
+        _native.ptr(net.packed), _native.ptr(net.params), _native.ptr(x),
+        _native.ptr(t), None, num, 1.0 / n, None, flags,
+        _native.ptr(net.grad), _native.ptr(net.loss), net.train_blocks,
+        _native.ptr(net.train_ws), net.train_ws.numel(),
+        _native.stream_handle()), "dr_mlp_train_step")
+
+  step(n, 0)  # accumulate onto the first pass
+  torch.testing.assert_close(net.grad, 2 * g1, rtol=1e-6, atol=1e-9)
+  torch.testing.assert_close(net.loss, 2 * l1, rtol=1e-6, atol=0)
+  step(n, _native.MLP_GRAD_OVERWRITE)
+  assert torch.equal(net.grad, g1) and torch.equal(net.loss, l1)
+  step(0, _native.MLP_GRAD_OVERWRITE)
+  assert not net.grad.any() and not net.loss.any()
+
+
 def test_mlp_adam_matches_oracle(device):
   p = mo.init_params(5)
   rng = np.random.default_rng(2)
