@@ -14,11 +14,15 @@ point_light(...)      -> shaders.point_light (:100-173), one light, as the
 MLPShader             12 -> 64x4 -> 3 MLP, bfloat16 MFMA, float32 accumulation,
                       trained by the library's own forward/backward/Adam
                       kernels; `shade_with_fallback` keeps the Phong result
-                      wherever the network's per-pixel error exceeds a bound.
+                      wherever the network's per-pixel error exceeds a bound;
+                      save / load keep weights and Adam state in a
+                      safetensors file.
 """
 import ctypes
 
 import numpy as np
+import safetensors
+import safetensors.torch as safetensors_torch
 import torch
 
 from diffren_amd import _native
@@ -256,6 +260,47 @@ class MLPShader:
     loss = self.loss_and_grad(x, target, mask, scale)
     self.adam_step(lr)
     return loss
+
+  def save(self, path):
+    """Writes the network and its training state to a safetensors file:
+    float32 master weights, Adam moments, the input normalisation and, as
+    metadata, the Adam step count and the albedo-modulation flag.  The
+    bfloat16 operand images are not stored (load re-packs them)."""
+    tensors = {"params": self.params, "adam_m": self.m, "adam_v": self.v}
+    if self.norm is not None:
+      tensors["input_norm"] = self.norm
+    tensors = {k: v.detach().cpu().contiguous() for k, v in tensors.items()}
+    meta = {
+        "format": "diffren_amd.MLPShader",
+        "layers": ",".join(str(s) for s in MLP_SIZES),
+        "step_count": str(self.step_count),
+        "albedo_modulate": str(int(bool(self.flags &
+                                        _native.MLP_ALBEDO_MODULATE))),
+    }
+    safetensors_torch.save_file(tensors, str(path), metadata=meta)
+
+  @classmethod
+  def load(cls, path, device, train_blocks=256):
+    """The shader saved by `save`, on `device`, ready to run or to train on
+    from the saved Adam state."""
+    with safetensors.safe_open(str(path), framework="pt") as f:
+      meta = f.metadata() or {}
+      if meta.get("format") != "diffren_amd.MLPShader":
+        raise ValueError(f"{path}: not a saved MLPShader")
+      layers = ",".join(str(s) for s in MLP_SIZES)
+      if meta.get("layers") != layers:
+        raise ValueError(f"{path}: layer sizes {meta.get('layers')} differ "
+                         f"from this library's {layers}")
+      t = {k: f.get_tensor(k) for k in f.keys()}
+    norm = t.get("input_norm")
+    mlp = cls(t["params"].numpy(), device,
+              input_norm=None if norm is None else norm.numpy(),
+              albedo_modulate=meta.get("albedo_modulate") == "1",
+              train_blocks=train_blocks)
+    mlp.m.copy_(t["adam_m"].to(device))
+    mlp.v.copy_(t["adam_v"].to(device))
+    mlp.step_count = int(meta.get("step_count", "0"))
+    return mlp
 
 
 def shade_with_fallback(mlp, gbuf, reference, tolerance=1e-2):

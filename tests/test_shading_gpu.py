@@ -235,3 +235,32 @@ def test_learned_shader_gain(device):
   assert err.mean().item() <= 5e-3
   assert fallback <= 0.05
   assert phong_ms >= 2.0 * mlp_ms
+
+
+def test_mlp_save_load_round_trip(device, tmp_path):
+  """A saved shader loads with the same weights, normalisation, flag and
+  Adam state: the same inference output, and the next training step gives
+  the same parameters as the original's."""
+  rng = np.random.default_rng(11)
+  n = 4096
+  x = torch.as_tensor(rng.normal(size=(n, 12)).astype(np.float32),
+                      device=device)
+  t = torch.as_tensor(rng.uniform(size=(n, 3)).astype(np.float32),
+                      device=device)
+  norm = shading.MLPShader.normalization(x)
+  net = shading.MLPShader(mo.init_params(4), device, input_norm=norm,
+                          albedo_modulate=True)
+  for _ in range(3):
+    net.train_step(x, t, lr=1e-3)
+  path = tmp_path / "shader.safetensors"
+  net.save(path)
+  back = shading.MLPShader.load(path, device)
+  assert back.step_count == net.step_count == 3
+  assert back.flags == net.flags
+  assert torch.equal(back.params, net.params)
+  assert torch.equal(back.norm, net.norm)
+  assert torch.equal(back.forward(x), net.forward(x))
+  net.train_step(x, t, lr=1e-3)
+  back.train_step(x, t, lr=1e-3)
+  assert torch.equal(back.params, net.params)
+  assert torch.equal(back.m, net.m) and torch.equal(back.v, net.v)
