@@ -22,6 +22,7 @@ RASTER_SPARSE = 1
 RASTER_BINS_ONLY = 2
 RASTER_TILES_ONLY = 4
 RENDER_ACCUM_BUFFERS = 4
+RENDER_SHADED = 8
 MLP_ALBEDO_MODULATE = 1
 MLP_GRAD_OVERWRITE = 2
 PAD_CONSTANT = 0
@@ -62,6 +63,7 @@ class SplatJob(ctypes.Structure):
               ("grad_scratch", _c_ptr), ("d_records", _c_ptr),
               ("accum", _c_ptr), ("norm", _c_ptr), ("d_camera", _c_ptr),
               ("accum_weights", _c_ptr),
+              ("shaded", _c_ptr), ("d_shaded", _c_ptr),
               ("num_vertices", _c_int),
               ("num_triangles", _c_int), ("width", _c_int),
               ("height", _c_int), ("tile_begin", _c_i64),
@@ -150,7 +152,7 @@ SIGNATURES = {
                                          _c_ptr, _c_ptr, _c_size, _c_ptr]),
     "dr_fixed_to_float": (_c_int, [_c_ptr, _c_i64, _c_int, _c_ptr, _c_ptr]),
     "dr_render_splat_backward": (_c_int, [_c_ptr, _c_int, _c_i64, _c_int,
-                                          _c_int, _c_ptr]),
+                                          _c_int, _c_int, _c_ptr]),
 }
 
 

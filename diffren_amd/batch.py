@@ -358,7 +358,7 @@ class BatchedSplatRenderer:
     _native.check(
         _native.lib().dr_render_splat_backward(
             _native.ptr(g.splat_table), len(g.indices), g.splat_tiles, self.L,
-            self.C, _native.stream_handle()), "dr_render_splat_backward")
+            self.C, 0, _native.stream_handle()), "dr_render_splat_backward")
 
   def rasterize(self):
     for g in self.groups:

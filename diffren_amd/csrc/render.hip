@@ -3,13 +3,14 @@
 //
 // Per job the inputs are the rasterizer's outputs (triangle ids, z, and
 // barycentrics, layer-major [L][H][W]), the clip-space vertices, the
-// triangles and per-vertex colours; the shading function is the identity on
-// the interpolated colours (render.py:123-132 with shading_function =
-// lambda a: a["colors"]).  What the reference computes as separate XLA passes
-// -- interpolate the colours (interpolate.py:24-63), interpolate clip xyzw
-// with background (0,0,1,1) and map to the viewport (splat.py:209-221),
-// splat_at_pixel_centers (splat.py:26-175) -- is done here without writing any
-// intermediate frame:
+// triangles and the colours to splat: per-vertex colours under an identity
+// shader (render.py:123-132 with shading_function = lambda a: a["colors"]),
+// or the shaded pixels [L][H][W][C] of any shading function
+// (splat_shaded_pixels' shaded_pixels, splat.py:178-229).  What the
+// reference computes as separate XLA passes -- interpolate the colours
+// (interpolate.py:24-63), interpolate clip xyzw with background (0,0,1,1)
+// and map to the viewport (splat.py:209-221), splat_at_pixel_centers
+// (splat.py:26-175) -- is done here without writing any intermediate frame:
 //
 //  target pass (one block per 64x4 tile of output pixels of one job):
 //    stage the tile plus a 1-pixel halo, all layers, in LDS: fg mask,
@@ -24,7 +25,9 @@
 //    form d colour and d viewport xy, map d xy to d clip (x, y, w), and
 //    scatter b_k-weighted gradients to the triangle's three vertices --
 //    d colours[v] and d vertices[v] = M^T d clip (or d clip itself when no
-//    camera is given).  Float atomics: summation order is not fixed.
+//    camera is given); with shaded pixels d colour is the pixel-layer's
+//    d shaded, written once, and only positions are scattered.  Float
+//    atomics: summation order is not fixed (a fixed-point mode fixes it).
 //
 // Viewport z is taken from the rasterizer's z buffer, ((z + 1) * 1) * 0.5:
 // the rasterizer computes z = ((b0 z0 + b1 z1) + b2 z2) / ((b0 w0 + b1 w1) +
@@ -366,12 +369,30 @@ void target_kernel(
       if (good) {
         float cx, cy, cw;
         splat_position<true>(J, k[j], px, py, cx, cy, cw);
-        const float* a0 = J.colors + int64_t(k[j].v[0]) * C;
-        const float* a1 = J.colors + int64_t(k[j].v[1]) * C;
-        const float* a2 = J.colors + int64_t(k[j].v[2]) * C;
+        if (J.shaded) {
+          // The shading function's output for this pixel-layer.
+          const int qx = x0 - 1 + i % kRX, qy = y0 - 1 + i / kRX;
+          const float* sp = J.shaded + (l * P + int64_t(qy) * W + qx) * C;
 #pragma unroll
-        for (int c = 0; c < C; ++c)
-          col[c] = interp3(k[j], ldg(a0 + c), ldg(a1 + c), ldg(a2 + c));
+          for (int c = 0; c < C; ++c) {
+            if constexpr (C % 4 == 0) {
+              if (c % 4 == 0) {
+                const float4 q4 = ldg4(sp + c);
+                col[c] = q4.x; col[c + 1] = q4.y; col[c + 2] = q4.z;
+                col[c + 3] = q4.w;
+              }
+            } else {
+              col[c] = ldg(sp + c);
+            }
+          }
+        } else {
+          const float* a0 = J.colors + int64_t(k[j].v[0]) * C;
+          const float* a1 = J.colors + int64_t(k[j].v[1]) * C;
+          const float* a2 = J.colors + int64_t(k[j].v[2]) * C;
+#pragma unroll
+          for (int c = 0; c < C; ++c)
+            col[c] = interp3(k[j], ldg(a0 + c), ldg(a1 + c), ldg(a2 + c));
+        }
       }
       sx[i] = px;
       sy[i] = py;
@@ -539,11 +560,32 @@ __global__ __launch_bounds__(1024) void loss_reduce_kernel(
 
 // Gradient record of one vertex: [R] floats, R = record_size(C):
 // cols 0..3 d position (x, y, z, 0) -- or d clip (x, y, 0, w) when the job
-// has no camera -- and cols 4 .. 4 + C - 1 d colour.
-template <int C>
+// has no camera -- and cols 4 .. 4 + C - 1 d colour.  Jobs with shaded
+// pixels scatter positions only: R = 4.  kBase is the number of leading
+// columns a fragment contributes to.
+template <int C, bool kShaded = false>
 struct Record {
-  static constexpr int kSize = (4 + C) <= 8 ? 8 : ((4 + C) <= 16 ? 16 : 32);
+  static constexpr int kBase = kShaded ? 4 : 4 + C;
+  static constexpr int kSize =
+      kShaded ? 4 : ((4 + C) <= 8 ? 8 : ((4 + C) <= 16 ? 16 : 32));
 };
+
+// Stores a pixel-layer's C-channel row (16-byte stores when C % 4 == 0);
+// v == nullptr stores zeros.
+template <int C>
+__device__ __forceinline__ void store_row(float* dst, const float* v) {
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    if constexpr (C % 4 == 0) {
+      if (c % 4 == 0)
+        *reinterpret_cast<float4*>(dst + c) =
+            v ? make_float4(v[c], v[c + 1], v[c + 2], v[c + 3])
+              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+      dst[c] = v ? v[c] : 0.0f;
+    }
+  }
+}
 
 // Source pass.  Float atomics to the vertex gradient records are the cost of
 // this pass (they run at a fixed rate of grouped operations), so it issues as
@@ -563,13 +605,13 @@ constexpr int kHashSlots = 256;   // > 3 x 64 entries per wave and layer
 constexpr int kRows = 96;
 
 // A fragment's record for its corner vertex v is b * base: base holds
-// (d position or d clip [4], d colour [C]); columns past 4 + C are zero.
+// (d position or d clip [4], d colour [C]); columns past kBase are zero.
 // Formed where it is used, so the three corners' records are never all
 // live in registers.
-template <int C>
+template <int kBase>
 __device__ __forceinline__ float record_entry(float b, const float* base,
                                               int c) {
-  return c < 4 + C ? b * base[c] : 0.0f;
+  return c < kBase ? b * base[c] : 0.0f;
 }
 
 // Adds v to a gradient entry: a float atomic, or -- in the deterministic
@@ -587,16 +629,15 @@ __device__ __forceinline__ void grad_add(float* p, int64_t i, float v,
   }
 }
 
-template <int C>
+template <int kBase, int R>
 __device__ __forceinline__ void flush_record(float* grads, int v, float b,
                                              const float* base,
                                              int fixed_shift,
                                              float fixed_scale) {
-  constexpr int R = Record<C>::kSize;
   if (grads == nullptr) return;
 #pragma unroll
   for (int c = 0; c < R; ++c) {
-    const float e = record_entry<C>(b, base, c);
+    const float e = record_entry<kBase>(b, base, c);
     if (e != 0.0f)
       grad_add(grads, int64_t(v) * R + c, e, fixed_shift, fixed_scale);
   }
@@ -610,9 +651,9 @@ __device__ __forceinline__ void flush_record(float* grads, int v, float b,
 // layer 0 has been differentiated, so the per-vertex rows reuse its space.
 constexpr int kPatchX = 18, kPatchY = 6, kPatch = kPatchX * kPatchY;
 
-template <int C>
+template <int C, bool kShaded>
 struct SourceLds {
-  static constexpr int R = Record<C>::kSize;
+  static constexpr int R = Record<C, kShaded>::kSize;
   static constexpr int GS = Scratch<C>::kStride;
   static constexpr bool kStageG = GS <= 8;
   static constexpr int kStageBytes =
@@ -626,11 +667,12 @@ struct SourceLds {
       ((kStageBytes > kRowBytes ? kStageBytes : kRowBytes) + 15) & ~15;
 };
 
-template <int C>
+template <int C, bool kShaded>
 __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     const dr_splat_job* __restrict__ jobs, int num_jobs, int L, float coef) {
-  constexpr int R = Record<C>::kSize;
-  using S = SourceLds<C>;
+  constexpr int R = Record<C, kShaded>::kSize;
+  constexpr int NB = Record<C, kShaded>::kBase;
+  using S = SourceLds<C, kShaded>;
   __shared__ float4 wave_lds[kBX * kBY / 64][S::kBytes / 16];
   __shared__ int hash_key[kBX * kBY / 64][kHashSlots];
   __shared__ int hash_row[kBX * kBY / 64][kHashSlots];
@@ -651,10 +693,14 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   const float4* stage_g = wave_lds[wave] + kPatch;
   int* hkey = hash_key[wave];
   int* hrow = hash_row[wave];
-  // Nothing to differentiate into, or no gradient scratch to read: the job
-  // contributes nothing (one job per block, so the whole block leaves).
-  if ((J.d_records == nullptr && J.d_camera == nullptr) ||
-      J.grad_scratch == nullptr)
+  // Nothing to differentiate into, no gradient scratch to read, or a job
+  // of the other colour source (this launch handles jobs with shaded pixels
+  // only, or jobs without): the job contributes nothing here (one job per
+  // block, so the whole block leaves).
+  float* const d_shaded = kShaded ? J.d_shaded : nullptr;
+  if ((J.d_records == nullptr && J.d_camera == nullptr &&
+       d_shaded == nullptr) ||
+      J.grad_scratch == nullptr || (J.shaded != nullptr) != kShaded)
     return;
   const int fshift = J.fixed_shift;
   const float fscale = fshift > 0 ? ldexpf(1.0f, fshift) : 1.0f;
@@ -670,7 +716,13 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   bool any = false;
   if (inside)
     for (int l = 0; l < L; ++l) any |= ldg(J.triangle_ids + l * P + p) >= 0;
-  if (__ballot(any) == 0) return;
+  if (__ballot(any) == 0) {
+    // d shaded is 0 wherever no fragment is (the fg mask zeroes the taps).
+    if (d_shaded && inside)
+      for (int l = 0; l < L; ++l)
+        store_row<C>(d_shaded + (l * P + p) * C, nullptr);
+    return;
+  }
   // Layer 0's corners are loaded now, so their loads are in flight together
   // with the patch staging below instead of after it.
   Corner k0;
@@ -728,19 +780,36 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   for (int l = 0; l < L; ++l) {
     Corner k = k0;
     const bool frag = l == 0 ? frag0 : any && corners(J, l * P + p, k);
-    float base[4 + C];
+    float base[NB];
     float dclip[3] = {0.0f, 0.0f, 0.0f};  // d clip x, y, w
+    float dcol[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) dcol[c] = 0.0f;
     if (frag) {
       float sx, sy, cx, cy, cw;
       splat_position<false>(J, k, sx, sy, cx, cy, cw);
-      const float* a0 = J.colors + int64_t(k.v[0]) * C;
-      const float* a1 = J.colors + int64_t(k.v[1]) * C;
-      const float* a2 = J.colors + int64_t(k.v[2]) * C;
-      float col[C], dcol[C];
+      float col[C];
+      if constexpr (kShaded) {
+        const float* sp = J.shaded + (l * P + p) * C;
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
-        col[c] = interp3(k, ldg(a0 + c), ldg(a1 + c), ldg(a2 + c));
-        dcol[c] = 0.0f;
+        for (int c = 0; c < C; ++c) {
+          if constexpr (C % 4 == 0) {
+            if (c % 4 == 0) {
+              const float4 q4 = ldg4(sp + c);
+              col[c] = q4.x; col[c + 1] = q4.y; col[c + 2] = q4.z;
+              col[c + 3] = q4.w;
+            }
+          } else {
+            col[c] = ldg(sp + c);
+          }
+        }
+      } else {
+        const float* a0 = J.colors + int64_t(k.v[0]) * C;
+        const float* a1 = J.colors + int64_t(k.v[1]) * C;
+        const float* a2 = J.colors + int64_t(k.v[2]) * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+          col[c] = interp3(k, ldg(a0 + c), ldg(a1 + c), ldg(a2 + c));
       }
       float gx = 0.0f, gy = 0.0f;
 #pragma unroll
@@ -803,8 +872,10 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
       }
 #pragma unroll
       for (int c = 0; c < 4; ++c) base[c] = dp[c];
+      if constexpr (!kShaded) {
 #pragma unroll
-      for (int c = 0; c < C; ++c) base[4 + c] = dcol[c];
+        for (int c = 0; c < C; ++c) base[4 + c] = dcol[c];
+      }
       dclip[0] = dcx;
       dclip[1] = dcy;
       dclip[2] = dcw;
@@ -834,6 +905,10 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
                      fscale);
         }
     }
+    // With shaded pixels the colour gradient belongs to the pixel-layer:
+    // written once, 0 where no fragment is.
+    if (d_shaded && inside)
+      store_row<C>(d_shaded + (l * P + p) * C, frag ? dcol : nullptr);
     // Wave-local dedupe into compact LDS rows (one per distinct vertex, in
     // the order first occurrences claim them), then a grouped flush.
     const uint64_t act = __ballot(frag);
@@ -882,9 +957,9 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
           vtx[r] = k.v[i];
 #pragma unroll
           for (int c = 0; c < R; ++c)
-            val[r * RS + c] = record_entry<C>(k.b[i], base, c);
+            val[r * RS + c] = record_entry<NB>(k.b[i], base, c);
         } else {
-          flush_record<C>(grads, k.v[i], k.b[i], base, fshift, fscale);
+          flush_record<NB, R>(grads, k.v[i], k.b[i], base, fshift, fscale);
         }
       }
       nr += __popcll(fm);
@@ -896,12 +971,12 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         if (first[i]) continue;
         const int r = hrow[slot[i]];
         if (r < 0) {
-          flush_record<C>(grads, k.v[i], k.b[i], base, fshift, fscale);
+          flush_record<NB, R>(grads, k.v[i], k.b[i], base, fshift, fscale);
           continue;
         }
 #pragma unroll
         for (int c = 0; c < R; ++c) {
-          const float e = record_entry<C>(k.b[i], base, c);
+          const float e = record_entry<NB>(k.b[i], base, c);
           if (e != 0.0f) atomicAdd(&val[r * RS + c], e);
         }
       }
@@ -975,9 +1050,13 @@ int launch_target(const dr_splat_job* jobs, int n, int64_t tiles, int L,
 
 template <int C>
 int launch_source(const dr_splat_job* jobs, int n, int64_t tiles, int L,
-                  hipStream_t s) {
-  source_kernel<C><<<unsigned(tiles), dim3(kBX, kBY), 0, s>>>(jobs, n, L,
-                                                              splat_coef());
+                  bool shaded, hipStream_t s) {
+  if (shaded)
+    source_kernel<C, true><<<unsigned(tiles), dim3(kBX, kBY), 0, s>>>(
+        jobs, n, L, splat_coef());
+  else
+    source_kernel<C, false><<<unsigned(tiles), dim3(kBX, kBY), 0, s>>>(
+        jobs, n, L, splat_coef());
   DR_CHECK_LAUNCH("render splat source pass");
   return kOk;
 }
@@ -1043,8 +1122,14 @@ int dr_splat_jobs_prepare(dr_splat_job* jobs, int32_t n, int32_t channels,
                  "null rasterizer output");
     DR_CHECK_ARG(j.num_triangles == 0 ||
                      ((j.clip_vertices || (j.positions && j.camera)) &&
-                      j.triangles && j.colors),
-                 "null mesh pointer (clip_vertices, or positions + camera)");
+                      j.triangles && (j.colors || j.shaded)),
+                 "null mesh pointer (clip_vertices, or positions + camera; "
+                 "colors or shaded)");
+    DR_CHECK_ARG(j.d_shaded == nullptr || j.shaded != nullptr,
+                 "d_shaded needs shaded");
+    DR_CHECK_ARG((reinterpret_cast<uintptr_t>(j.shaded) & 15) == 0 &&
+                     (reinterpret_cast<uintptr_t>(j.d_shaded) & 15) == 0,
+                 "shaded and d_shaded must be 16-byte aligned");
     DR_CHECK_ARG((reinterpret_cast<uintptr_t>(j.clip_vertices) & 15) == 0,
                  "clip_vertices must be 16-byte aligned");
     DR_CHECK_ARG((reinterpret_cast<uintptr_t>(j.grad_scratch) & 15) == 0,
@@ -1124,13 +1209,17 @@ int dr_fixed_to_float(const int64_t* fixed, int64_t n, int32_t fixed_shift,
 int dr_render_splat_backward(const dr_splat_job* jobs_device,
                              int32_t num_jobs, int64_t total_tiles,
                              int32_t num_layers, int32_t channels,
-                             dr_stream_t stream) {
+                             int32_t flags, dr_stream_t stream) {
   int rc = dr::check(num_jobs, total_tiles, num_layers, channels);
   if (rc != DR_OK) return rc;
   DR_CHECK_ARG(jobs_device != nullptr, "jobs is null");
+  DR_CHECK_ARG((flags & ~DR_RENDER_SHADED) == 0,
+               "flags must be 0 or DR_RENDER_SHADED");
+  const bool shaded = (flags & DR_RENDER_SHADED) != 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
-#define DR_CALL(CC) \
-  dr::launch_source<CC>(jobs_device, num_jobs, total_tiles, num_layers, s)
+#define DR_CALL(CC)                                                        \
+  dr::launch_source<CC>(jobs_device, num_jobs, total_tiles, num_layers, \
+                        shaded, s)
   DR_RENDER_SWITCH(channels, DR_CALL)
 #undef DR_CALL
   return dr::fail(dr::kUnsupported, "unsupported channel count");

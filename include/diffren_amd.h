@@ -312,10 +312,12 @@ int dr_splat_bwd(const float* xyz, const float* color, const float* mask,
 /* Fused, batched rasterize-then-splat render (forward + backward)          */
 /* ------------------------------------------------------------------------ */
 
-/* One render job: a rasterized view of a mesh with per-vertex colours and an
- * identity shader (render.py:123-132 with shading_function returning the
- * interpolated colours).  All pointers are device pointers; C = channels,
- * the last channel is alpha. */
+/* One render job: a rasterized view of a mesh and the colours to splat --
+ * either per-vertex `colors` under an identity shader (render.py:123-132
+ * with shading_function returning the interpolated colours), or the shaded
+ * pixels of any shading function (`shaded`, splat.py:178-229's
+ * shaded_pixels).  All pointers are device pointers; C = channels, the last
+ * channel is alpha. */
 typedef struct dr_splat_job {
   const float* clip_vertices;   /* [V, 4] clip space, 16-byte aligned; or
                                    NULL to transform `positions` by `camera`
@@ -353,6 +355,15 @@ typedef struct dr_splat_job {
                                    (accum and norm are the reference's
                                    return_accum_buffers outputs,
                                    splat.py:166-175) */
+  const float* shaded;          /* [L, H, W, C] shaded pixels, or NULL for
+                                   per-vertex `colors`.  When set, `colors`
+                                   is not read, d_records carry only the
+                                   position gradient (R = 4) and the colour
+                                   gradient goes to d_shaded. */
+  float* d_shaded;              /* [L, H, W, C] gradient of the loss with
+                                   respect to `shaded`, written completely
+                                   by the backward (0 on background), or
+                                   NULL */
   int32_t num_vertices;
   int32_t num_triangles;
   int32_t width;
@@ -379,10 +390,11 @@ int dr_splat_jobs_prepare(dr_splat_job* jobs_host, int32_t num_jobs,
 int32_t dr_render_scratch_floats(int32_t channels);
 
 /* Floats per vertex gradient record for `channels` colour channels (8, 16
- * or 32; 0 if unsupported). */
+ * or 32; 0 if unsupported).  Jobs with `shaded` pixels use 4-float records
+ * (the position gradient only). */
 int32_t dr_render_record_size(int32_t channels);
 
-enum { DR_RENDER_ACCUM_BUFFERS = 4 };
+enum { DR_RENDER_ACCUM_BUFFERS = 4, DR_RENDER_SHADED = 8 };
 
 /* Workspace bytes of dr_render_splat_forward (per-tile loss partials). */
 size_t dr_render_splat_workspace_size(int64_t total_tiles);
@@ -414,14 +426,17 @@ int dr_fixed_to_float(const int64_t* fixed, int64_t n, int32_t fixed_shift,
                       float* out, dr_stream_t stream);
 
 /* Source pass: from grad_scratch (written by a forward with grad_mode 1 or
- * 2), accumulate the d_records of all jobs (float atomics); jobs with a NULL
- * d_records or grad_scratch are skipped.  The
- * hand-written VJP of splat.py:178-229 + interpolate.py:24-63 +
- * transforms.py:40-78; the reference uses XLA autodiff. */
+ * 2), accumulate the d_records (and d_camera) of all jobs (float atomics, or
+ * fixed point) and write the d_shaded of jobs with shaded pixels; jobs with
+ * no grad_scratch, or none of d_records, d_camera and d_shaded, are
+ * skipped.  The hand-written VJP of splat.py:178-229 + interpolate.py:24-63
+ * + transforms.py:40-78; the reference uses XLA autodiff.  flags 0
+ * differentiates the jobs with per-vertex colours, DR_RENDER_SHADED the jobs
+ * with shaded pixels (the others are skipped). */
 int dr_render_splat_backward(const dr_splat_job* jobs_device,
                              int32_t num_jobs, int64_t total_tiles,
                              int32_t num_layers, int32_t channels,
-                             dr_stream_t stream);
+                             int32_t flags, dr_stream_t stream);
 
 
 /* ------------------------------------------------------------------------ */

@@ -445,7 +445,7 @@ def test_jobs_without_gradient_inputs(device):
         _native.ptr(loss), _native.ptr(ws), ws.numel(),
         _native.stream_handle()), "forward")
     _native.check(lib.dr_render_splat_backward(
-        _native.ptr(dev_table), 1, tiles.value, 3, r.C,
+        _native.ptr(dev_table), 1, tiles.value, 3, r.C, 0,
         _native.stream_handle()), "backward")
     torch.cuda.synchronize()
     assert torch.equal(image, want)
