@@ -10,8 +10,11 @@ to vertex positions and colours.  Inputs are resident in HBM before timing.
 
 Reported next to it, measured in the same run:
   * the plain path: one render_triangles call per view (one rasterizer launch
-    per image, unfused interpolate / splat kernels, autograd backward), the
-    pattern of the reference's vmap_method='sequential_unrolled';
+    per image, then transform, interpolation, viewport and splat as separate
+    passes, autograd backward), the pattern of the reference's
+    vmap_method='sequential_unrolled' and of its separate XLA passes;
+  * the per-view fused path: the same calls with the splat on the fused
+    render kernels (render_triangles' default), one image per launch;
   * per-stage kernel times from HIP events and the roofline of the dominant
     kernel;
   * the CPU baseline: the reference CPU rasterizer (compiled from the
@@ -189,7 +192,7 @@ def pmc_probe(args):
   print("pmc probe done", flush=True)
 
 
-def plain_step(w, args):
+def plain_step(w, args, fused=False):
   import torch
   from diffren_amd import render
   vv = w["mesh"]["vertices"].clone().requires_grad_(True)
@@ -199,7 +202,7 @@ def plain_step(w, args):
     img = render.render_triangles(vv, {"c": cc}, w["mesh"]["triangles"],
                                   w["cams_t"][i], args.resolution,
                                   args.resolution, lambda a: a["c"],
-                                  num_layers=args.layers)
+                                  num_layers=args.layers, fused_splat=fused)
     loss = ((img - w["targets"][i]) ** 2).sum() / npix
     loss.backward()
   return vv.grad, cc.grad
@@ -359,7 +362,7 @@ def grad_check(w, args, dev):
     plain = render.render_triangles(
         w["mesh"]["vertices"], {"c": w["mesh"]["colors"]},
         w["mesh"]["triangles"], w["cams_t"][0], R, R, lambda a: a["c"],
-        num_layers=L)
+        num_layers=L, fused_splat=False)
   image_err = rel(r.images[0], out32.double())
   plain_image_err = rel(plain, out32.double())
   image_err64 = rel(r.images[0], out64)
@@ -510,16 +513,21 @@ def main():
                   binning_half_ms=round(launch_ms["raster_bins"], 4),
                   kernels=kernels)
 
-  # Plain path: one render_triangles per view + autograd (one warm-up step).
-  plain_step(w, args)
-  torch.cuda.synchronize()
-  t0 = time.perf_counter()
-  for _ in range(args.plain_steps):
-    plain_step(w, args)
-  torch.cuda.synchronize()
-  plain_dt = (time.perf_counter() - t0) / args.plain_steps
-  plain_rps = B / plain_dt
-  log(f"plain: {plain_dt * 1e3:.3f} ms/step, {plain_rps:.1f} renders/s")
+  # Per-view paths: one render_triangles per view + autograd (one warm-up
+  # step each): unfused passes (plain), and the splat on the fused kernels.
+  per_view_rps = {}
+  for fused in (False, True):
+    plain_step(w, args, fused)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.plain_steps):
+      plain_step(w, args, fused)
+    torch.cuda.synchronize()
+    pdt = (time.perf_counter() - t0) / args.plain_steps
+    per_view_rps[fused] = B / pdt
+    log(f"per view ({'fused splat' if fused else 'plain'}): "
+        f"{pdt * 1e3:.3f} ms/step, {B / pdt:.1f} renders/s")
+  plain_rps = per_view_rps[False]
 
   extra = {}
   if not args.no_grad_check:
@@ -559,6 +567,9 @@ def main():
       "mtri_per_s": round(renders_per_s * T / 1e6, 1),
       "plain_renders_per_s": round(plain_rps, 2),
       "speedup_vs_plain": round(renders_per_s / plain_rps, 2),
+      "per_view_fused_renders_per_s": round(per_view_rps[True], 2),
+      "speedup_vs_per_view_fused": round(renders_per_s / per_view_rps[True],
+                                         2),
       "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
       "roofline": roofline,
       "cpu_baseline": cpu,

@@ -9,8 +9,9 @@ all 64 jobs + L2 loss against targets rendered from perturbed copies of the
 meshes + backward to every mesh's vertex positions and colours, run by
 BatchedSplatRenderer (one launch per stage for all jobs, tile shape and bin
 capacity from the autotuner, captured in a HIP graph).  Reported next to it:
-the same work as one render_triangles call plus autograd backward per job
-(the plain path).  Prints one JSON line.
+the same work as one render_triangles call plus autograd backward per job,
+with the splat as separate passes (the plain path) and on the fused kernels
+(per-view fused).  Prints one JSON line.
 """
 import argparse
 import json
@@ -68,18 +69,20 @@ def run(args):
   r.capture()
   fused_ms = timed(r.replay, args.steps)
 
-  def plain():
+  def run_per_view(fused):
     for i, s in enumerate(work):
       m = meshes[i]
       vv = m["vertices"].clone().requires_grad_(True)
       cc = m["colors"].clone().requires_grad_(True)
       res = s["resolution"]
       img = render.render_triangles(vv, {"c": cc}, m["triangles"], jobs[i][1],
-                                    res, res, lambda a: a["c"], num_layers=L)
+                                    res, res, lambda a: a["c"], num_layers=L,
+                                    fused_splat=fused)
       loss = ((img - targets[i]) ** 2).sum() / (res * res * 4)
       loss.backward()
 
-  plain_ms = timed(plain, args.plain_steps)
+  plain_ms = timed(lambda: run_per_view(False), args.plain_steps)
+  per_view_fused_ms = timed(lambda: run_per_view(True), args.plain_steps)
   n = len(jobs)
   return {
       "workload": (f"{n} meshes, {tris} triangles in total (log-uniform "
@@ -93,6 +96,8 @@ def run(args):
       "plain_ms_per_step": round(plain_ms, 4),
       "plain_renders_per_s": round(n / (plain_ms * 1e-3), 1),
       "speedup_vs_plain": round(plain_ms / fused_ms, 2),
+      "per_view_fused_ms_per_step": round(per_view_fused_ms, 4),
+      "speedup_vs_per_view_fused": round(per_view_fused_ms / fused_ms, 2),
       "raster_tile": [r.raster_config.tile_w, r.raster_config.tile_h],
       "bin_capacity": int(r.raster_config.bin_capacity),
       "streams": args.streams,

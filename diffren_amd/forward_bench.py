@@ -6,8 +6,9 @@ plus 8 per-vertex features ~ N(0, 1) and alpha 1 as a pass-through shader
 (C = Cs = 12), rendered at 1024x1024 with 3 layers from 8 seeded poses on a
 radius-3 orbit.  One step = rasterize + interpolate + splat + over of all 8
 views.  The batched fused path (BatchedSplatRenderer.render captured in a
-HIP graph) is timed next to the plain path (render_triangles per view, no
-autograd), and the fused target kernel's launches are timed one by one for
+HIP graph) is timed next to the per-view paths (render_triangles per view,
+no autograd: unfused passes -- the plain path -- and with the splat on the
+fused kernels), and the fused target kernel's launches are timed one by one for
 its algorithmic rate: L*P*16 (ids + barycentrics) + T*12 + V*4*C read and
 P*4*Cs written per view, against the 8 TB/s HBM peak.  Prints one JSON line.
 """
@@ -79,13 +80,15 @@ def run(args):
               P * 4 * C)
   achieved = per_view * views_per_launch / (launch_ms * 1e-3) / 1e9
 
-  def plain():
+  def run_per_view(fused):
     with torch.no_grad():
       for c in cams:
         render.render_triangles(vt, {"a": attrs}, tt, c, R, R,
-                                lambda a: a["a"], num_layers=L)
+                                lambda a: a["a"], num_layers=L,
+                                fused_splat=fused)
 
-  plain_ms = timed(plain, args.plain_steps)
+  plain_ms = timed(lambda: run_per_view(False), args.plain_steps)
+  per_view_fused_ms = timed(lambda: run_per_view(True), args.plain_steps)
   n = len(jobs)
   return {
       "workload": (f"1M-triangle Perlin sphere ({t.shape[0]} triangles), "
@@ -97,6 +100,8 @@ def run(args):
       "plain_ms_per_step": round(plain_ms, 4),
       "plain_renders_per_s": round(n / (plain_ms * 1e-3), 1),
       "speedup_vs_plain": round(plain_ms / fused_ms, 2),
+      "per_view_fused_ms_per_step": round(per_view_fused_ms, 4),
+      "speedup_vs_per_view_fused": round(per_view_fused_ms / fused_ms, 2),
       "target_kernel": {
           "kernel": f"target_kernel<{C}>", "avg_launch_ms": round(launch_ms, 4),
           "views_per_launch": views_per_launch,
