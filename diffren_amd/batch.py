@@ -17,12 +17,19 @@ This is the batched counterpart of calling render.render_triangles(...,
 shading_function=lambda a: a["colors"], compositing_mode=SPLAT_OVER) once per
 view (render.py:28-140 under jax.vmap, which issues one rasterizer launch per
 image: rasterize_triangles_xla.py:68-75) followed by jax.grad of an L2 loss.
+With `shading_function` set, any shader runs between the rasterizer and the
+fused splat: each job's mesh attributes are interpolated (interp kernels),
+shaded by the function (torch, differentiable), splatted from the shaded
+pixels by the fused kernels, and the shaded pixels' gradient is pushed back
+through the shader and the interpolation by autograd.
 """
 import ctypes
 
 import torch
 
 from diffren_amd import _native
+from diffren_amd import framebuffer
+from diffren_amd import interpolate
 from diffren_amd import rasterize
 
 
@@ -67,6 +74,19 @@ class BatchedSplatRenderer:
            the GPU's atomics arrive, so repeated steps give identical bits.
            The loss is always summed in a fixed order.
 
+  shading_function: None for the identity shader on the meshes' "colors".
+           Otherwise a function of a dict of interpolated attributes
+           ([L, H, W, Ca] each, the meshes' "attributes" dicts of [V, Ca]
+           device tensors, background 0) returning [L, H, W, channels]
+           shaded pixels, alpha last -- render_triangles' shading_function
+           (render.py:123-129).  The step then interpolates, shades, splats
+           the shaded pixels with the fused kernels, and back-propagates the
+           shaded pixels' gradient through the shader and the interpolation
+           (torch.autograd.grad): d_attributes[m][name] holds the gradient
+           with respect to mesh m's attribute `name` (for the attributes
+           that require grad), d_vertices the gradient through the splat
+           positions, d_colors is None.  `channels` is then required.
+
   After step(): images[i], loss (sum over jobs of loss_scale * |img - tgt|^2),
   d_vertices[m] [V,3] and d_colors[m] [V,C] (views of records[m]).  ids[i]
   holds triangle ids; z[i] / barys[i] are defined under triangles only.
@@ -75,23 +95,43 @@ class BatchedSplatRenderer:
   def __init__(self, meshes, jobs, num_layers, tile=None, bin_capacity=0,
                loss_scale=None, autotune_cache=None, streams=1,
                fused_transform=False, return_accum_buffers=False,
-               camera_grads=False, deterministic=False, fixed_shift=40):
+               camera_grads=False, deterministic=False, fixed_shift=40,
+               shading_function=None, channels=None):
     lib = _native.lib()
     self.meshes = meshes
     self.jobs = jobs
     self.L = int(num_layers)
     dev = meshes[0]["vertices"].device
     self.device = dev
-    C = meshes[0]["colors"].shape[1]
+    self.shading_function = shading_function
+    if shading_function is not None:
+      if channels is None:
+        raise ValueError("shading_function needs `channels`, the channel "
+                         "count of its output")
+      C = int(channels)
+    else:
+      C = meshes[0]["colors"].shape[1]
     for m in meshes:
-      v, t, c = m["vertices"], m["triangles"], m["colors"]
-      _native.require_device(v, t, c)
+      v, t = m["vertices"], m["triangles"]
+      _native.require_device(v, t)
       if (v.dtype != torch.float32 or v.dim() != 2 or v.shape[1] != 3 or
           not v.is_contiguous()):
         raise ValueError("mesh vertices must be contiguous float32 [V, 3]")
       if (t.dtype != torch.int32 or t.dim() != 2 or t.shape[1] != 3 or
           not t.is_contiguous()):
         raise ValueError("mesh triangles must be contiguous int32 [T, 3]")
+      if shading_function is not None:
+        attrs = m.get("attributes")
+        if not isinstance(attrs, dict) or not attrs:
+          raise ValueError("with a shading_function every mesh needs an "
+                           "\"attributes\" dict of [V, Ca] tensors")
+        for a in attrs.values():
+          _native.require_device(a)
+          if a.dim() != 2 or a.shape[0] != v.shape[0]:
+            raise ValueError("mesh attributes must be [V, Ca]")
+        continue
+      c = m["colors"]
+      _native.require_device(c)
       if (c.dtype != torch.float32 or c.dim() != 2 or
           c.shape[0] != v.shape[0] or not c.is_contiguous()):
         raise ValueError("mesh colors must be contiguous float32 [V, C]")
@@ -108,6 +148,8 @@ class BatchedSplatRenderer:
     # Per-mesh gradient records [V, R]: d position in cols 0..2, d colour in
     # cols 4..4+C-1 (one contiguous record per vertex for grouped atomics).
     R = lib.dr_render_record_size(C)
+    if R > 0 and shading_function is not None:
+      R = 4  # shaded pixels: the records carry the position gradient only
     if R <= 0:
       raise ValueError(
           f"the fused path takes 1-16, 20, 24 or 28 colour channels, not {C}:"
@@ -117,7 +159,8 @@ class BatchedSplatRenderer:
                                 dtype=torch.float32, device=dev)
                     for m in meshes]
     self.d_vertices = [r[:, 0:3] for r in self.records]
-    self.d_colors = [r[:, 4:4 + C] for r in self.records]
+    self.d_colors = (None if shading_function is not None else
+                     [r[:, 4:4 + C] for r in self.records])
     self.fixed_shift = int(fixed_shift) if deterministic else 0
     if deterministic and not 1 <= self.fixed_shift <= 62:
       raise ValueError("fixed_shift must be in [1, 62]")
@@ -172,6 +215,18 @@ class BatchedSplatRenderer:
       self.scratch.append(torch.empty((h, w, sf), dtype=torch.float32,
                                       device=dev))
       npix += w * h * C
+    self.shaded, self.d_shaded = [], []
+    self.d_attributes = None
+    if shading_function is not None:
+      self.d_attributes = [
+          {k: torch.zeros(a.shape, dtype=torch.float32, device=dev)
+           for k, a in m["attributes"].items() if a.requires_grad}
+          for m in meshes]
+      for _, _, w, h in jobs:
+        self.shaded.append(torch.zeros((self.L, h, w, C), dtype=torch.float32,
+                                       device=dev))
+        self.d_shaded.append(torch.zeros((self.L, h, w, C),
+                                         dtype=torch.float32, device=dev))
     self.loss_scale = 1.0 / npix if loss_scale is None else float(loss_scale)
     self.cameras = [j[1].to(torch.float32).contiguous() for j in jobs]
     self.autotune_cache = autotune_cache
@@ -286,7 +341,11 @@ class BatchedSplatRenderer:
         j.d_camera = (self.d_cameras_fixed if self.d_cameras_fixed is not None
                       else self.d_cameras)[i].data_ptr()
       j.triangles = m["triangles"].data_ptr()
-      j.colors = m["colors"].data_ptr()
+      if self.shading_function is not None:
+        j.shaded = self.shaded[i].data_ptr()
+        j.d_shaded = self.d_shaded[i].data_ptr()
+      else:
+        j.colors = m["colors"].data_ptr()
       j.camera = self.cameras[i].data_ptr()
       j.triangle_ids = self.ids[i].data_ptr()
       j.z_buffer = self.z[i].data_ptr()
@@ -358,7 +417,9 @@ class BatchedSplatRenderer:
     _native.check(
         _native.lib().dr_render_splat_backward(
             _native.ptr(g.splat_table), len(g.indices), g.splat_tiles, self.L,
-            self.C, 0, _native.stream_handle()), "dr_render_splat_backward")
+            self.C,
+            _native.RENDER_SHADED if self.shading_function is not None else 0,
+            _native.stream_handle()), "dr_render_splat_backward")
 
   def rasterize(self):
     for g in self.groups:
@@ -403,26 +464,85 @@ class BatchedSplatRenderer:
     self.loss_parts.zero_()
     self.loss.zero_()
 
-  def _group_step(self, g, grad_mode):
+  def _group_raster(self, g):
     self._rasterize(g)
     self._status_host[self.groups.index(g)].copy_(g.raster_ws[:16],
                                                   non_blocking=True)
+
+  def _group_splat(self, g, grad_mode):
     self._splat_forward(g, grad_mode)
     if grad_mode:
       self._splat_backward(g)
 
-  def _run_groups(self, grad_mode):
+  def _group_step(self, g, grad_mode):
+    self._group_raster(g)
+    self._group_splat(g, grad_mode)
+
+  def _on_streams(self, fn):
+    """fn(group) for every group, each on its own stream (forked from and
+    joined back into the current stream)."""
     if not self.streams:
-      self._group_step(self.groups[0], grad_mode)
+      fn(self.groups[0])
+      return
+    cur = torch.cuda.current_stream()
+    for s in self.streams:
+      s.wait_stream(cur)
+    for g, s in zip(self.groups, self.streams):
+      with torch.cuda.stream(s):
+        fn(g)
+    for s in self.streams:
+      cur.wait_stream(s)
+
+  def _shade(self, grad):
+    """Interpolates every job's attributes, runs the shading function and
+    stores its output in the job's shaded buffer; returns the outputs that
+    carry gradients (for the backward)."""
+    outs = []
+    with torch.set_grad_enabled(grad):
+      for i, (mi, _, w, h) in enumerate(self.jobs):
+        m = self.meshes[mi]
+        fb = framebuffer.Framebuffer(
+            barycentrics=self.barys[i], triangle_id=self.ids[i].unsqueeze(-1),
+            triangles=m["triangles"], z_buffer=self.z[i])
+        # Background 0 (render.py:124), written by the kernel itself (no
+        # background tensor: nothing is copied from the host, so the step
+        # can be graph-captured).
+        attrs = interpolate.interpolate_vertex_attributes(m["attributes"], fb)
+        out = self.shading_function(attrs)
+        if tuple(out.shape) != (self.L, h, w, self.C):
+          raise ValueError(
+              f"shading_function returned {tuple(out.shape)} for job {i}, "
+              f"expected {(self.L, h, w, self.C)}")
+        self.shaded[i].copy_(out.detach())
+        if grad and out.requires_grad:
+          outs.append((out, self.d_shaded[i]))
+    return outs
+
+  def _attribute_grads(self, outs):
+    """d_attributes from the shaded pixels' gradient (written by the source
+    pass into d_shaded) through the shader and the interpolation."""
+    want = [(mi, k, a) for mi, m in enumerate(self.meshes)
+            for k, a in m["attributes"].items() if a.requires_grad]
+    if not want:
+      return
+    grads = (torch.autograd.grad([o for o, _ in outs], [a for _, _, a in want],
+                                 [d for _, d in outs], allow_unused=True)
+             if outs else [None] * len(want))
+    for (mi, k, _), g in zip(want, grads):
+      if g is None:
+        self.d_attributes[mi][k].zero_()
+      else:
+        self.d_attributes[mi][k].copy_(g)
+
+  def _run_groups(self, grad_mode):
+    if self.shading_function is None:
+      self._on_streams(lambda g: self._group_step(g, grad_mode))
     else:
-      cur = torch.cuda.current_stream()
-      for s in self.streams:
-        s.wait_stream(cur)
-      for g, s in zip(self.groups, self.streams):
-        with torch.cuda.stream(s):
-          self._group_step(g, grad_mode)
-      for s in self.streams:
-        cur.wait_stream(s)
+      self._on_streams(self._group_raster)
+      outs = self._shade(grad_mode != 0)
+      self._on_streams(lambda g: self._group_splat(g, grad_mode))
+      if grad_mode:
+        self._attribute_grads(outs)
     if grad_mode == 1:
       self._sum_loss()
     if grad_mode:

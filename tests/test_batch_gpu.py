@@ -504,3 +504,107 @@ def test_bin_overflow_regrows(device):
   torch.cuda.synchronize()
   assert r.regrows == 1
   assert torch.equal(r.images[0], want)
+
+
+_LIGHT = {}
+
+
+def _point_light(attrs):
+  """RGBA point-light shader on interpolated colour / normal / position
+  (shaders.point_light, shaders.py:100-173), alpha interpolated.  Its
+  constants live on the device (created once, outside any graph capture)."""
+  from diffren_amd import shading
+  dev = attrs["color"].device
+  if dev not in _LIGHT:
+    _LIGHT[dev] = (torch.tensor([1.0, 2.5, 3.0], device=dev),
+                   torch.zeros(3, device=dev))
+  light, eye = _LIGHT[dev]
+  rgb = shading.point_light(attrs["color"][..., :3],
+                            shading.l2_normalize(attrs["normal"]),
+                            attrs["position"], light, eye_position=eye,
+                            ks=0.5, ns=8.0)
+  return torch.cat([rgb, attrs["color"][..., 3:]], -1)
+
+
+def _shaded_meshes(device):
+  from diffren_amd import mesh
+  meshes = _meshes(device)
+  for m in meshes:
+    m["attributes"] = {
+        "position": m["vertices"].clone().requires_grad_(True),
+        "normal": mesh.compute_vertex_normals(m["vertices"],
+                                              m["triangles"]),
+        "color": m["colors"].clone().requires_grad_(True)}
+  return meshes
+
+
+@pytest.mark.parametrize("streams", [1, 4])
+def test_batched_shader_matches_per_view(streams, device):
+  """BatchedSplatRenderer with a point-light shading_function against
+  render_triangles per view with the same shader (splat on the fused
+  kernels) and against the unfused per-view path: images, loss, the
+  position gradient through the splat and the attribute gradients through
+  the shader; then graph replays against the eager step."""
+  meshes = _shaded_meshes(device)
+  jobs = _jobs(meshes)
+  L = 3
+  r = batch.BatchedSplatRenderer(meshes, jobs, L, streams=streams,
+                                 shading_function=_point_light, channels=4)
+  assert r.d_colors is None
+  gen = torch.Generator(device="cpu").manual_seed(11)
+  for t in r.targets:
+    t.copy_(torch.rand(t.shape, generator=gen))
+  r.step()
+  torch.cuda.synchronize()
+  npix = sum(w * h * 4 for _, _, w, h in jobs)
+  for fused in (None, False):
+    vs = [m["vertices"].clone().requires_grad_(True) for m in meshes]
+    leaves = [{k: a.detach().clone().requires_grad_(a.requires_grad)
+               for k, a in m["attributes"].items()} for m in meshes]
+    total = 0.0
+    for i, ((mi, cam, w, h), tgt) in enumerate(zip(jobs, r.targets)):
+      img = render.render_triangles(vs[mi], leaves[mi],
+                                    meshes[mi]["triangles"], cam, w, h,
+                                    _point_light, num_layers=L,
+                                    fused_splat=fused)
+      tol = 0.0 if fused is None else 1e-5
+      assert helpers.rel_err(r.images[i].cpu(), img.detach().cpu()) <= tol
+      total = total + ((img - tgt) ** 2).sum() / npix
+    total.backward()
+    assert abs(r.loss.item() - total.item()) <= 1e-5 * abs(total.item())
+    for mi in range(len(meshes)):
+      assert helpers.rel_err(r.d_vertices[mi].cpu(), vs[mi].grad.cpu()) <= 1e-4
+      for k in ("position", "color"):
+        assert helpers.rel_err(r.d_attributes[mi][k].cpu(),
+                               leaves[mi][k].grad.cpu()) <= 1e-4, (mi, k)
+      assert "normal" not in r.d_attributes[mi]
+  want = [t.clone() for t in r.images + r.d_vertices] + [
+      d[k].clone() for d in r.d_attributes for k in sorted(d)]
+  want_loss = r.loss.item()
+  r.capture()
+  for k in range(3):
+    r.replay()
+    torch.cuda.synchronize()
+    got = r.images + r.d_vertices + [d[n] for d in r.d_attributes
+                                     for n in sorted(d)]
+    for a, b in zip(got, want):
+      assert helpers.rel_err(a.cpu(), b.cpu()) <= 1e-5, k
+    assert abs(r.loss.item() - want_loss) <= 1e-5 * abs(want_loss)
+    torch.ones(1000, device=device).sum().item()
+
+
+def test_batched_shader_forward_only(device):
+  """render() with a shading_function: images equal the step's."""
+  meshes = _shaded_meshes(device)
+  jobs = _jobs(meshes)
+  r = batch.BatchedSplatRenderer(meshes, jobs, 2,
+                                 shading_function=_point_light, channels=4)
+  r.render()
+  torch.cuda.synchronize()
+  imgs = [t.clone() for t in r.images]
+  r.step()
+  torch.cuda.synchronize()
+  for a, b in zip(imgs, r.images):
+    assert torch.equal(a, b)
+  with pytest.raises(ValueError):
+    batch.BatchedSplatRenderer(meshes, jobs, 2, shading_function=_point_light)
