@@ -593,10 +593,16 @@ __device__ __forceinline__ void store_row(float* dst, const float* v) {
 //  * each wave covers a 16x4 patch of pixels, whose fragments reference each
 //    mesh vertex ~4 times on average (neighbouring pixels' triangles share
 //    vertices);
-//  * the wave finds repeated vertices among its fragments' three vertex
-//    records with a small LDS hash table, keeps one LDS row per distinct
-//    vertex (the first occurrence writes it, repeats add into it) and
-//    flushes only those rows;
+//  * neighbouring pixels of a row often lie on the same triangle (an active
+//    wave's ~62 fragments lie on ~36 triangles at 1M triangles / 1024^2):
+//    the corner records of a run of such lanes are summed into the run's
+//    first lane by a segmented sum over the row (DPP), and only that lane
+//    goes on -- this also takes away the same-address LDS atomics of lanes
+//    on one triangle;
+//  * the wave finds repeated vertices among the remaining three vertex
+//    records per lane with a small LDS hash table, keeps one LDS row per
+//    distinct vertex (the first occurrence writes it, repeats add into it)
+//    and flushes only those rows;
 //  * the flush puts R consecutive lanes on one record, because grouped
 //    atomics run ~7x faster than one scattered address per lane.
 constexpr int kHashSlots = 256;   // > 3 x 64 entries per wave and layer
@@ -606,12 +612,43 @@ constexpr int kRows = 96;
 
 // A fragment's record for its corner vertex v is b * base: base holds
 // (d position or d clip [4], d colour [C]); columns past kBase are zero.
-// Formed where it is used, so the three corners' records are never all
-// live in registers.
+// Without the run merge (wide records) it is formed where it is used, so
+// the three corners' records are never all live in registers.
 template <int kBase>
 __device__ __forceinline__ float record_entry(float b, const float* base,
                                               int c) {
   return c < kBase ? b * base[c] : 0.0f;
+}
+
+// x of the lane kD places to the right in this lane's row of 16 lanes (DPP
+// row_shl:kD), or `fallback` past the row's end.
+template <int kD>
+__device__ __forceinline__ int row_from_right(int x, int fallback) {
+  return __builtin_amdgcn_update_dpp(fallback, x, 0x100 + kD, 0xf, 0xf,
+                                     false);
+}
+template <int kD>
+__device__ __forceinline__ float row_from_right(float x) {
+  return __int_as_float(row_from_right<kD>(__float_as_int(x), 0));
+}
+
+// One doubling step of a segmented suffix sum over rows of 16 lanes: a lane
+// whose run (seg: the lane index where the run starts) continues kD lanes to
+// the right adds that lane's partial sums, which then cover up to 2 kD lanes
+// of the run.  Returns false when no lane of the wave takes anything (then
+// no run is longer than kD and later steps would take nothing either).
+template <int kD, int N>
+__device__ __forceinline__ bool run_sum_step(int seg, float (&x)[3][N]) {
+  const bool take = row_from_right<kD>(seg, -1) == seg;
+  if (__ballot(take) == 0) return false;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      const float r = row_from_right<kD>(x[i][c]);
+      x[i][c] += take ? r : 0.0f;
+    }
+  return true;
 }
 
 // Adds v to a gradient entry: a float atomic, or -- in the deterministic
@@ -629,15 +666,16 @@ __device__ __forceinline__ void grad_add(float* p, int64_t i, float v,
   }
 }
 
-template <int kBase, int R>
-__device__ __forceinline__ void flush_record(float* grads, int v, float b,
-                                             const float* base,
+// Adds a corner's record (entry(c), c < R) straight to vertex v's gradient.
+template <int R, typename Entry>
+__device__ __forceinline__ void flush_record(float* grads, int v,
+                                             const Entry& entry,
                                              int fixed_shift,
                                              float fixed_scale) {
   if (grads == nullptr) return;
 #pragma unroll
   for (int c = 0; c < R; ++c) {
-    const float e = record_entry<kBase>(b, base, c);
+    const float e = entry(c);
     if (e != 0.0f)
       grad_add(grads, int64_t(v) * R + c, e, fixed_shift, fixed_scale);
   }
@@ -913,12 +951,45 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     // the order first occurrences claim them), then a grouped flush.
     const uint64_t act = __ballot(frag);
     if (act == 0) continue;
+    // Corner records, summed over runs of lanes on one triangle when they
+    // are narrow enough to keep all three in registers (the runs' first
+    // lanes then carry the entries; the other lanes drop out).  Below four
+    // channels the kernel holds five waves per SIMD without the merge and
+    // four with it, so it is left out there.
+    constexpr bool kMergeRuns = NB <= 8 && C >= 4;
+    float rec[3][kMergeRuns ? NB : 1];
+    bool ent = frag;
+    if constexpr (kMergeRuns) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          rec[i][c] = frag ? k.b[i] * base[c] : 0.0f;
+      // Lanes without a fragment get distinct negative keys: runs of one.
+      const int tri = frag ? ldg(J.triangle_ids + l * P + p) : -1 - lane;
+      const int left = __builtin_amdgcn_update_dpp(
+          int(0x80000000), tri, 0x111, 0xf, 0xf, false);  // row_shr:1
+      const bool starts = (lane & 15) == 0 || left != tri;
+      const uint64_t upto = ~uint64_t(0) >> (63 - lane);
+      const int seg = 63 - __builtin_clzll(__ballot(starts) & upto);
+      ent = frag && starts;
+      if (run_sum_step<1>(seg, rec))
+        if (run_sum_step<2>(seg, rec))
+          if (run_sum_step<4>(seg, rec)) run_sum_step<8>(seg, rec);
+    }
+    // Column c of corner i's record.
+    auto entry = [&](int i, int c) -> float {
+      if constexpr (kMergeRuns)
+        return c < NB ? rec[i][c] : 0.0f;
+      else
+        return record_entry<NB>(k.b[i], base, c);
+    };
     for (int i = lane; i < kHashSlots; i += 64) hkey[i] = -1;
     wave_sync();
     float* grads = J.d_records;
     int slot[3] = {0, 0, 0};
     bool first[3] = {false, false, false};
-    if (frag) {
+    if (ent) {
       // Linear probing; the table has more slots than a wave has entries,
       // so every probe sequence ends at a free slot or a match.  The three
       // vertices' first probes are issued back to back and collisions walk
@@ -948,7 +1019,7 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     int nr = 0;  // rows taken so far (wave-uniform)
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const bool f = frag && first[i];
+      const bool f = ent && first[i];
       const uint64_t fm = __ballot(f);
       if (f) {
         const int r = nr + __popcll(fm & lanemask_lt());
@@ -957,26 +1028,28 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
           vtx[r] = k.v[i];
 #pragma unroll
           for (int c = 0; c < R; ++c)
-            val[r * RS + c] = record_entry<NB>(k.b[i], base, c);
+            val[r * RS + c] = entry(i, c);
         } else {
-          flush_record<NB, R>(grads, k.v[i], k.b[i], base, fshift, fscale);
+          flush_record<R>(grads, k.v[i], [&](int c) { return entry(i, c); },
+                          fshift, fscale);
         }
       }
       nr += __popcll(fm);
     }
     wave_sync();
-    if (frag) {
+    if (ent) {
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         if (first[i]) continue;
         const int r = hrow[slot[i]];
         if (r < 0) {
-          flush_record<NB, R>(grads, k.v[i], k.b[i], base, fshift, fscale);
+          flush_record<R>(grads, k.v[i], [&](int c) { return entry(i, c); },
+                          fshift, fscale);
           continue;
         }
 #pragma unroll
         for (int c = 0; c < R; ++c) {
-          const float e = record_entry<NB>(k.b[i], base, c);
+          const float e = entry(i, c);
           if (e != 0.0f) atomicAdd(&val[r * RS + c], e);
         }
       }
