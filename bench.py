@@ -18,8 +18,9 @@ Reported next to it, measured in the same run:
   * per-stage kernel times from HIP events and the roofline of the dominant
     kernel;
   * the CPU baseline: the reference CPU rasterizer (compiled from the
-    reference sources into oracle/_ref) plus the float32 CPU restatement of
-    interpolate/splat forward+backward, on one view;
+    reference sources into oracle/_ref, one view per host thread) plus the
+    float32 C restatement of interpolate/splat forward+backward on all host
+    threads, for the workload's views;
   * the vertex-gradient relative error of one view against the float64 CPU
     oracle.
 """
@@ -498,7 +499,7 @@ def main():
       "raster_tiles": "raster_tile_kernel<%d, %d>" % (
           r.raster_config.tile_w, r.raster_config.tile_h),
       "splat_target": "target_kernel<4, false>",
-      "splat_source": "source_kernel<4>",
+      "splat_source": "source_kernel<4, false>",
   }
   launch_ms = {k: time_launches(r, k, reps)
                for k in ("raster_bins", "raster_tiles", "splat_target",
