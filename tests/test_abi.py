@@ -101,14 +101,14 @@ def test_render_channel_counts_and_job_checks():
     assert lib.dr_render_record_size(c) == 0
     assert lib.dr_render_scratch_floats(c) == 0
   job = (_native.SplatJob * 1)()
-  job[0].width = 8
-  job[0].height = 8
+  job[0].width = 40
+  job[0].height = 20
   job[0].triangle_ids = 256
   job[0].z_buffer = 256
   job[0].barycentrics = 256
   tiles = ctypes.c_int64()
   assert lib.dr_splat_jobs_prepare(job, 1, 4, ctypes.byref(tiles)) == 0
-  assert tiles.value == 2 * 1  # 64x4 tiles: one column, two rows
+  assert tiles.value == 2 * 3  # 32x8 tiles: two columns, three rows
   assert lib.dr_splat_jobs_prepare(job, 1, 17, ctypes.byref(tiles)) != 0
   assert b"channel" in lib.dr_last_error()
   job[0].grad_scratch = 260
