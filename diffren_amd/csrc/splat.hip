@@ -32,6 +32,9 @@ namespace dr {
 namespace {
 
 constexpr int kBX = 64, kBY = 4;
+// Frames of up to kMaxLayers layers keep a pixel's layer depths in
+// registers; deeper frames (any L, as splat.py:26-175 takes) run a variant
+// that reads them from memory where the surface index needs them.
 constexpr int kMaxLayers = 8;
 
 struct SplatFrame {
@@ -42,31 +45,66 @@ struct SplatFrame {
   float coef;
 };
 
-// Surface index of the (source S, target T) pair; splat.py:131-145.
-__device__ __forceinline__ int surface_index(const float* zS, const float* zT,
+// Surface index of the (source S, target T) pair; splat.py:131-145.  zS(l)
+// and zT(l) give layer l's z of the source and the target pixel.
+template <typename ZS, typename ZT>
+__device__ __forceinline__ int surface_index(const ZS& zS, const ZT& zT,
                                              int L) {
-  float bc = fabsf(zS[0] - zT[0]);
+  const float s0 = zS(0), t0 = zT(0);
+  float bc = fabsf(s0 - t0);
   int ic = 0;
-  float bt = fabsf(zT[0] - zS[0]);
+  float bt = fabsf(t0 - s0);
   int it = 0;
   for (int l = 1; l < L; ++l) {
-    const float dc = fabsf(zS[l] - zT[0]);
+    const float dc = fabsf(zS(l) - t0);
     if (dc < bc) { bc = dc; ic = l; }
-    const float dt = fabsf(zT[l] - zS[0]);
+    const float dt = fabsf(zT(l) - s0);
     if (dt < bt) { bt = dt; it = l; }
   }
   return bt < bc ? -it : ic;
 }
 
-__device__ __forceinline__ int layer_class(int surface, int l) {
-  return surface > l ? 0 : (surface == l ? 1 : 2);
+// Layer l's z at pixel p, from memory.
+__device__ __forceinline__ float frame_z(const SplatFrame& f, int l,
+                                         int64_t p) {
+  return f.xyz[3 * (l * int64_t(f.H) * f.W + p) + 2];
 }
 
-__device__ __forceinline__ void load_z(const SplatFrame& f, int x, int y,
-                                       float* z) {
-  const int64_t P = int64_t(f.H) * f.W;
+// Surface indices of pixel (x, y) with its 9 neighbours, tap order
+// t = 3 (u + 1) + (v + 1).  As a target (kTarget) the neighbour at
+// (x - u, y - v) is the source; otherwise (x, y) is the source and
+// (x + u, y + v) the target.  Out-of-image neighbours get 0 (never read).
+template <bool kAnyL, bool kTarget>
+__device__ __forceinline__ void surface_indices(const SplatFrame& f, int x,
+                                                int y, int surf[9]) {
   const int64_t p = int64_t(y) * f.W + x;
-  for (int l = 0; l < f.L; ++l) z[l] = f.xyz[3 * (l * P + p) + 2];
+  float zp[kAnyL ? 1 : kMaxLayers], zn[kAnyL ? 1 : kMaxLayers];
+  if constexpr (!kAnyL)
+    for (int l = 0; l < f.L; ++l) zp[l] = frame_z(f, l, p);
+#pragma unroll
+  for (int t = 0; t < 9; ++t) {
+    const int u = t / 3 - 1, v = t % 3 - 1;
+    const int nx = kTarget ? x - u : x + u, ny = kTarget ? y - v : y + v;
+    surf[t] = 0;
+    if (nx < 0 || nx >= f.W || ny < 0 || ny >= f.H) continue;
+    const int64_t n = int64_t(ny) * f.W + nx;
+    if constexpr (kAnyL) {
+      auto here = [&](int l) { return frame_z(f, l, p); };
+      auto there = [&](int l) { return frame_z(f, l, n); };
+      surf[t] = kTarget ? surface_index(there, here, f.L)
+                        : surface_index(here, there, f.L);
+    } else {
+      for (int l = 0; l < f.L; ++l) zn[l] = frame_z(f, l, n);
+      auto here = [&](int l) { return zp[l]; };
+      auto there = [&](int l) { return zn[l]; };
+      surf[t] = kTarget ? surface_index(there, here, f.L)
+                        : surface_index(here, there, f.L);
+    }
+  }
+}
+
+__device__ __forceinline__ int layer_class(int surface, int l) {
+  return surface > l ? 0 : (surface == l ? 1 : 2);
 }
 
 // Tap weight and offset d = floor(xy) + 0.5 - xy + (u, v).
@@ -80,7 +118,7 @@ __device__ __forceinline__ float tap_weight(float m, float coef, float sx,
 }
 
 // acc[k][0..C) colour, acc[k][C] weight, for target (x, y).
-template <int C>
+template <int C, bool kAnyL>
 __device__ __forceinline__ void accumulate_target(const SplatFrame& f, int x,
                                                   int y,
                                                   float acc[3][C + 1]) {
@@ -89,18 +127,8 @@ __device__ __forceinline__ void accumulate_target(const SplatFrame& f, int x,
 #pragma unroll
     for (int c = 0; c <= C; ++c) acc[k][c] = 0.0f;
   const int64_t P = int64_t(f.H) * f.W;
-  float zT[kMaxLayers], zS[kMaxLayers];
-  load_z(f, x, y, zT);
   int surf[9];
-#pragma unroll
-  for (int t = 0; t < 9; ++t) {
-    const int u = t / 3 - 1, v = t % 3 - 1;
-    const int sx = x - u, sy = y - v;
-    surf[t] = 0;
-    if (sx < 0 || sx >= f.W || sy < 0 || sy >= f.H) continue;
-    load_z(f, sx, sy, zS);
-    surf[t] = surface_index(zS, zT, f.L);
-  }
+  surface_indices<kAnyL, true>(f, x, y, surf);
   for (int l = 0; l < f.L; ++l) {
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
@@ -134,7 +162,7 @@ __device__ __forceinline__ float norm_scale(float w) {
   return 1.0f / (fmaxf(w - 1.0f, 0.0f) + 1.0f);
 }
 
-template <int C>
+template <int C, bool kAnyL>
 __global__ __launch_bounds__(kBX* kBY) void splat_fwd_kernel(
     SplatFrame f, float* __restrict__ out, float* __restrict__ accum,
     float* __restrict__ norm, float* __restrict__ accum_w) {
@@ -143,7 +171,7 @@ __global__ __launch_bounds__(kBX* kBY) void splat_fwd_kernel(
   const int y = tb.y * kBY + threadIdx.y;
   if (x >= f.W || y >= f.H) return;
   float acc[3][C + 1];
-  accumulate_target<C>(f, x, y, acc);
+  accumulate_target<C, kAnyL>(f, x, y, acc);
   const int64_t P = int64_t(f.H) * f.W;
   const int64_t p = int64_t(y) * f.W + x;
   float n[3][C];
@@ -174,7 +202,7 @@ __global__ __launch_bounds__(kBX* kBY) void splat_fwd_kernel(
 }
 
 // Pass 1 of the backward: G[p][k][0..C] = d acc (colour, weight).
-template <int C>
+template <int C, bool kAnyL>
 __global__ __launch_bounds__(kBX* kBY) void splat_bwd_target_kernel(
     SplatFrame f, const float* __restrict__ d_out, float* __restrict__ G) {
   const int2 tb = xcd_block_2d<16>();
@@ -182,7 +210,7 @@ __global__ __launch_bounds__(kBX* kBY) void splat_bwd_target_kernel(
   const int y = tb.y * kBY + threadIdx.y;
   if (x >= f.W || y >= f.H) return;
   float acc[3][C + 1];
-  accumulate_target<C>(f, x, y, acc);
+  accumulate_target<C, kAnyL>(f, x, y, acc);
   const int64_t p = int64_t(y) * f.W + x;
   float s[3], n[3][C];
 #pragma unroll
@@ -223,7 +251,7 @@ __global__ __launch_bounds__(kBX* kBY) void splat_bwd_target_kernel(
 }
 
 // Pass 2 of the backward: per source pixel, all layers.
-template <int C>
+template <int C, bool kAnyL>
 __global__ __launch_bounds__(kBX* kBY) void splat_bwd_source_kernel(
     SplatFrame f, const float* __restrict__ G, float* __restrict__ d_color,
     float* __restrict__ d_xyz) {
@@ -233,18 +261,8 @@ __global__ __launch_bounds__(kBX* kBY) void splat_bwd_source_kernel(
   if (x >= f.W || y >= f.H) return;
   const int64_t P = int64_t(f.H) * f.W;
   const int64_t p = int64_t(y) * f.W + x;
-  float zS[kMaxLayers], zT[kMaxLayers];
-  load_z(f, x, y, zS);
   int surf[9];
-#pragma unroll
-  for (int t = 0; t < 9; ++t) {
-    const int u = t / 3 - 1, v = t % 3 - 1;
-    const int tx = x + u, ty = y + v;
-    surf[t] = 0;
-    if (tx < 0 || tx >= f.W || ty < 0 || ty >= f.H) continue;
-    load_z(f, tx, ty, zT);
-    surf[t] = surface_index(zS, zT, f.L);
-  }
+  surface_indices<kAnyL, false>(f, x, y, surf);
   for (int l = 0; l < f.L; ++l) {
     const int64_t q = l * P + p;
     const float m = f.mask[q];
@@ -297,8 +315,12 @@ template <int C>
 int launch_fwd(const SplatFrame& f, float* out, float* accum, float* norm,
                float* accum_w, hipStream_t s) {
   dim3 grid((f.W + kBX - 1) / kBX, (f.H + kBY - 1) / kBY);
-  splat_fwd_kernel<C><<<grid, dim3(kBX, kBY), 0, s>>>(f, out, accum, norm,
-                                                      accum_w);
+  if (f.L <= kMaxLayers)
+    splat_fwd_kernel<C, false><<<grid, dim3(kBX, kBY), 0, s>>>(
+        f, out, accum, norm, accum_w);
+  else
+    splat_fwd_kernel<C, true><<<grid, dim3(kBX, kBY), 0, s>>>(
+        f, out, accum, norm, accum_w);
   DR_CHECK_LAUNCH("splat fwd");
   return kOk;
 }
@@ -307,10 +329,19 @@ template <int C>
 int launch_bwd(const SplatFrame& f, const float* d_out, float* G,
                float* d_color, float* d_xyz, hipStream_t s) {
   dim3 grid((f.W + kBX - 1) / kBX, (f.H + kBY - 1) / kBY);
-  splat_bwd_target_kernel<C><<<grid, dim3(kBX, kBY), 0, s>>>(f, d_out, G);
-  DR_CHECK_LAUNCH("splat bwd target");
-  splat_bwd_source_kernel<C><<<grid, dim3(kBX, kBY), 0, s>>>(f, G, d_color,
-                                                             d_xyz);
+  if (f.L <= kMaxLayers) {
+    splat_bwd_target_kernel<C, false><<<grid, dim3(kBX, kBY), 0, s>>>(
+        f, d_out, G);
+    DR_CHECK_LAUNCH("splat bwd target");
+    splat_bwd_source_kernel<C, false><<<grid, dim3(kBX, kBY), 0, s>>>(
+        f, G, d_color, d_xyz);
+  } else {
+    splat_bwd_target_kernel<C, true><<<grid, dim3(kBX, kBY), 0, s>>>(
+        f, d_out, G);
+    DR_CHECK_LAUNCH("splat bwd target");
+    splat_bwd_source_kernel<C, true><<<grid, dim3(kBX, kBY), 0, s>>>(
+        f, G, d_color, d_xyz);
+  }
   DR_CHECK_LAUNCH("splat bwd source");
   return kOk;
 }
@@ -346,7 +377,7 @@ bool splat_channels_ok(int c) {
 }
 
 int check_frame(const SplatFrame& f, int C) {
-  DR_CHECK_ARG(f.L >= 1 && f.L <= kMaxLayers, "num_layers must be in [1, 8]");
+  DR_CHECK_ARG(f.L >= 1, "num_layers must be >= 1");
   DR_CHECK_ARG(f.H > 0 && f.W > 0, "image size must be > 0");
   DR_CHECK_ARG(splat_channels_ok(C),
                "splat channel count must be 1-16, 20, 24, 28 or 32");

@@ -288,7 +288,8 @@ float dr_splat_coefficient(void);
  *   out [H,W,C]; optional (NULL to skip): accum [3,H,W,C], norm [3,H,W,C],
  *   accum_weights [3,H,W].  C one of 1-16, 20, 24, 28, 32 (callers pad
  *   other counts with zero channels before the alpha channel, or split
- *   them), L in [1, 8]. */
+ *   them), any L >= 1 (up to 8 layers a pixel's layer depths stay in
+ *   registers; deeper frames read them from memory where needed). */
 int dr_splat_fwd(const float* xyz, const float* color, const float* mask,
                  int32_t num_layers, int32_t height, int32_t width,
                  int32_t channels, float* out, float* accum, float* norm,

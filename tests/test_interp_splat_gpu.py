@@ -136,6 +136,69 @@ def test_splat_backward_vs_float64_autodiff(layers, channels, device):
   assert helpers.rel_err(x.grad.cpu()[..., :2], x64.grad[..., :2]) <= GRAD_TOL
 
 
+def _deep_stack_inputs(seed, w, h, layers, channels):
+  """Pixel-layer samples with many layers: positions jittered around the
+  pixel centres, z increasing front to back, each pixel's stack cut at a
+  random depth (z 1 and mask 0 below it, as a rasterizer leaves them)."""
+  rng = np.random.default_rng(seed)
+  xs, ys = np.meshgrid(np.arange(w) + 0.5, np.arange(h) + 0.5)
+  xyz = np.empty((layers, h, w, 3), np.float32)
+  xyz[..., 0] = xs + rng.uniform(-0.3, 0.3, size=(layers, h, w))
+  xyz[..., 1] = ys + rng.uniform(-0.3, 0.3, size=(layers, h, w))
+  xyz[..., 2] = np.sort(rng.uniform(0.1, 0.9, size=(layers, h, w)), axis=0)
+  depth = rng.integers(0, layers + 1, size=(h, w))
+  mask = (np.arange(layers)[:, None, None] < depth[None]).astype(np.float32)
+  xyz[..., 2] = np.where(mask > 0, xyz[..., 2], 1.0)
+  color = rng.uniform(0, 1, size=(layers, h, w, channels)).astype(np.float32)
+  return xyz, color, mask[..., None]
+
+
+# 8 layers keep the depths in registers; 9 and 12 take the kernels that read
+# them from memory (any number of layers, as splat.py:26-175 takes).
+@pytest.mark.parametrize("layers,channels", [(8, 4), (9, 3), (12, 4)])
+def test_splat_deep_stacks(layers, channels, device):
+  xyz, color, mask = _deep_stack_inputs(layers, 26, 19, layers, channels)
+  x = torch.as_tensor(xyz, device=device).requires_grad_(True)
+  c = torch.as_tensor(color, device=device).requires_grad_(True)
+  out, acc, norm = splat.splat_at_pixel_centers(
+      x, c, torch.as_tensor(mask, device=device))
+  g = np.random.default_rng(1).normal(size=out.shape).astype(np.float32)
+  out.backward(torch.as_tensor(g, device=device))
+  w_out, w_acc, w_norm = so.splat_at_pixel_centers(
+      torch.as_tensor(xyz), torch.as_tensor(color), torch.as_tensor(mask))
+  assert helpers.rel_err(out.detach().cpu(), w_out) <= FWD_TOL
+  assert helpers.rel_err(acc.cpu(), w_acc) <= FWD_TOL
+  assert helpers.rel_err(norm.cpu(), w_norm) <= FWD_TOL
+  x64 = torch.as_tensor(xyz, dtype=torch.float64).requires_grad_(True)
+  c64 = torch.as_tensor(color, dtype=torch.float64).requires_grad_(True)
+  o64, a64, _ = so.splat_at_pixel_centers(
+      x64, c64, torch.as_tensor(mask, dtype=torch.float64))
+  # The deeper layers reach every class: fg, surface and bg all get weight.
+  assert (a64[..., -1] > 0).flatten(1).any(1).all()
+  o64.backward(torch.as_tensor(g, dtype=torch.float64))
+  assert helpers.rel_err(c.grad.cpu(), c64.grad) <= GRAD_TOL
+  assert helpers.rel_err(x.grad.cpu()[..., :2], x64.grad[..., :2]) <= GRAD_TOL
+
+
+def test_render_more_layers_than_the_fused_kernels(device):
+  """render_triangles with 10 layers: the splat takes the unfused
+  composition (the fused kernels stop at 8), the result matches the
+  oracle."""
+  s = scenes.icosphere_scene(seed=4)
+  w, h, L = 48, 40, 10
+  img, v, col = _render_gpu(s, w, h, L, device)
+  img.sum().backward()
+  clip = so.transform(s["camera"], s["vertices"], torch.float32).numpy()
+  ids, _, b = raster_oracle.rasterize(clip, s["triangles"], w, h, L, 1)
+  v64 = torch.as_tensor(s["vertices"], dtype=torch.float64).requires_grad_()
+  c64 = torch.as_tensor(s["colors"], dtype=torch.float64).requires_grad_()
+  o64 = so.render_splat(v64, c64, s["camera"], ids, b, s["triangles"])
+  o64.sum().backward()
+  assert helpers.rel_err(img.detach().cpu(), o64.detach()) <= FWD_TOL
+  assert helpers.rel_err(v.grad.cpu(), v64.grad) <= GRAD_TOL
+  assert helpers.rel_err(col.grad.cpu(), c64.grad) <= GRAD_TOL
+
+
 def _render_gpu(s, w, h, layers, device, colors=None):
   v = torch.as_tensor(s["vertices"], device=device).requires_grad_(True)
   col = torch.as_tensor(s["colors"] if colors is None else colors,
