@@ -340,7 +340,8 @@ def grad_check(w, args, dev):
   R, L = args.resolution, args.layers
   r = batch.BatchedSplatRenderer([w["mesh"]], [(0, w["cams_t"][0], R, R)], L,
                                  loss_scale=1.0 / (R * R * 4),
-                                 return_accum_buffers=True)
+                                 return_accum_buffers=True,
+                                 deterministic=args.deterministic)
   r.targets[0].copy_(w["targets"][0])
   r.step()
   torch.cuda.synchronize()
