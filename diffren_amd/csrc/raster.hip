@@ -509,48 +509,53 @@ __global__ __launch_bounds__(kBlock) void bin_count_kernel(
     e.ntx = tiles_x(job, TW);
     float4 v[3];
     Box b;
-    if (load_triangle(job, e.t, v) &&
-        triangle_bbox(v, job.width, job.height, b)) {
-      int64_t m[9];
+    int64_t m[9];
+    // The exact cull first: it needs only the fixed-point matrix, and the
+    // triangles it rejects (most back faces) then skip the clipped box and
+    // its six divisions.  Either test alone means no entries.
+    bool live = load_triangle(job, e.t, v);
+    if (live) {
       if (small_coords(v)) fixed_matrix_small(v, m); else fixed_matrix(v, m);
-      if (!cannot_cover(m, cull)) {
-        bool whole = true;
-        if (b.right - b.left <= kSmallSide && b.top - b.bottom <= kSmallSide) {
-          // The tile pass's arithmetic exactly (edge_setup's adjugate of the
-          // same fixed-point matrix, shade_pixel).
-          int64_t r[9];
-          adjugate(m, cull, r);
-          const float sx2 = 2.0f / static_cast<float>(job.width);
-          const float sy2 = 2.0f / static_cast<float>(job.height);
-          int nf = 0;
-          uint32_t offs = 0;
-          uint32_t z[kInlineFrags] = {0u, 0u, 0u, 0u, 0u, 0u};
-          for (int iy = b.bottom; iy < b.top; ++iy)
-            for (int ix = b.left; ix < b.right; ++ix) {
-              Fragment f;
-              if (!shade_pixel(ix, iy, sx2, sy2, r, v, f)) continue;
-              if (nf < kInlineFrags) {
-                offs |= uint32_t(((iy - b.bottom) << 2) | (ix - b.left))
-                        << (4 * nf);
+      live = !cannot_cover(m, cull) &&
+             triangle_bbox(v, job.width, job.height, b);
+    }
+    if (live) {
+      bool whole = true;
+      if (b.right - b.left <= kSmallSide && b.top - b.bottom <= kSmallSide) {
+        // The tile pass's arithmetic exactly (edge_setup's adjugate of the
+        // same fixed-point matrix, shade_pixel).
+        int64_t r[9];
+        adjugate(m, cull, r);
+        const float sx2 = 2.0f / static_cast<float>(job.width);
+        const float sy2 = 2.0f / static_cast<float>(job.height);
+        int nf = 0;
+        uint32_t offs = 0;
+        uint32_t z[kInlineFrags] = {0u, 0u, 0u, 0u, 0u, 0u};
+        for (int iy = b.bottom; iy < b.top; ++iy)
+          for (int ix = b.left; ix < b.right; ++ix) {
+            Fragment f;
+            if (!shade_pixel(ix, iy, sx2, sy2, r, v, f)) continue;
+            if (nf < kInlineFrags) {
+              offs |= uint32_t(((iy - b.bottom) << 2) | (ix - b.left))
+                      << (4 * nf);
 #pragma unroll
-                for (int k = 0; k < kInlineFrags; ++k)
-                  if (k == nf) z[k] = __float_as_uint(f.z);
-              }
-              ++nf;
+              for (int k = 0; k < kInlineFrags; ++k)
+                if (k == nf) z[k] = __float_as_uint(f.z);
             }
-          if (nf <= kInlineFrags) {
-            whole = false;
-            e.n = nf;
-            if (nf > 0) {
-              e.w0 = kHeadFrag | uint32_t(nf) | (offs << 3);
-              e.w1 = uint32_t(b.left) | (uint32_t(b.bottom) << 16);
-              frag_z[2 * flat] = make_uint4(z[0], z[1], z[2], z[3]);
-              if (nf > 4) frag_z[2 * flat + 1] = make_uint4(z[4], z[5], 0u, 0u);
-            }
+            ++nf;
+          }
+        if (nf <= kInlineFrags) {
+          whole = false;
+          e.n = nf;
+          if (nf > 0) {
+            e.w0 = kHeadFrag | uint32_t(nf) | (offs << 3);
+            e.w1 = uint32_t(b.left) | (uint32_t(b.bottom) << 16);
+            frag_z[2 * flat] = make_uint4(z[0], z[1], z[2], z[3]);
+            if (nf > 4) frag_z[2 * flat + 1] = make_uint4(z[4], z[5], 0u, 0u);
           }
         }
-        if (whole) whole_triangle<TW, TH>(job, b, e);
       }
+      if (whole) whole_triangle<TW, TH>(job, b, e);
     }
     head[flat] = make_uint2(e.w0, e.w1);
   }
