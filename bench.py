@@ -36,6 +36,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def _groups_arg(v):
+  return v if v == "auto" else int(v)
+
+
 def parse():
   p = argparse.ArgumentParser()
   p.add_argument("--gpus", type=int, default=1)
@@ -50,8 +54,9 @@ def parse():
   p.add_argument("--no-cpu-baseline", action="store_true")
   p.add_argument("--no-grad-check", action="store_true")
   p.add_argument("--no-graph", action="store_true")
-  p.add_argument("--streams", type=int, default=4,
-                 help="concurrent job groups in the fused step")
+  p.add_argument("--streams", type=_groups_arg, default="auto",
+                 help="concurrent job groups in the fused step, or 'auto' "
+                 "(the autotuner times 1, 2 and 4 groups)")
   p.add_argument("--deterministic", action="store_true",
                  help="fixed-point (order-independent) gradient sums")
   p.add_argument("--fused-transform", action="store_true",
@@ -176,8 +181,11 @@ def pmc_probe(args):
   w = build_workload(args, dev)
   R, L = args.resolution, args.layers
   jobs = [(0, w["cams_t"][i], R, R) for i in range(args.views)]
+  # Four fixed groups of two views when not given: the counters' per-launch
+  # figures then cover two views, whatever the autotuner would pick.
   r = batch.BatchedSplatRenderer([w["mesh"]], jobs, L, tile=(16, 16),
-                                 streams=args.streams,
+                                 streams=(4 if args.streams == "auto" else
+                                          args.streams),
                                  fused_transform=args.fused_transform)
   for t in r.targets:
     t.uniform_()
@@ -440,7 +448,8 @@ def main():
                                  deterministic=args.deterministic)
   log(f"autotuned raster tile {r.raster_config.tile_w}x"
       f"{r.raster_config.tile_h}, bin capacity {r.raster_config.bin_capacity}"
-      f", timings {r.autotune_timings}")
+      f", timings {r.autotune_timings}; {len(r.groups)} job groups"
+      f" (step ms by group count: {r.streams_timings})")
   for i in range(B):
     r.targets[i].copy_(w["targets"][i])
   r.step()
@@ -571,14 +580,18 @@ def main():
       "data": "synthetic (seeded Perlin-displaced UV sphere, seeded cameras)",
       "config": {"workload": "1M-triangle sphere, 1024x1024, 3 layers, 8 "
                              "views, RGBA, fwd + L2 loss + bwd, batched "
-                             f"fused path on {args.streams} stream(s)" +
+                             f"fused path, {len(r.groups)} job groups on "
+                             f"as many streams" +
+                             (" (autotuned)" if args.streams == "auto"
+                              else "") +
                              (", transform on load" if args.fused_transform
                               else "") +
                              (", deterministic gradients" if
                               args.deterministic else "") +
                              (" in a HIP graph" if use_graph else ""),
                  "triangles": T, "vertices": V, "resolution": R, "layers": L,
-                 "views": B, "channels": C, "parallelism": "single GPU"},
+                 "views": B, "channels": C, "job_groups": len(r.groups),
+                 "parallelism": "single GPU"},
       "mtri_per_s": round(renders_per_s * T / 1e6, 1),
       "plain_renders_per_s": round(plain_rps, 2),
       "speedup_vs_plain": round(renders_per_s / plain_rps, 2),

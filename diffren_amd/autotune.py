@@ -1,12 +1,15 @@
-"""Autotuner for the tile rasterizer: screen-tile shape and bin capacity.
+"""Autotuner for the batched render: screen-tile shape, bin capacity and the
+number of concurrent job groups.
 
 For a batch of raster jobs it times every candidate tile shape with HIP
 events on the current stream (after one warm-up launch), keeps the fastest,
 then sizes the bin capacity from the entry count that configuration actually
 needed (dr_raster_bin_stats) plus headroom, so the default over-allocation
-(3 entries per triangle) shrinks to what the workload uses.  Results are cached
-in memory per workload signature (image sizes, triangle counts, layers,
-culling) and optionally in a JSON file.
+(3 entries per triangle) shrinks to what the workload uses.  tune_groups
+times the whole captured step for each candidate number of job groups (each
+group's kernels run on their own stream, so groups overlap) and keeps the
+fastest.  Results are cached in memory per workload signature (image sizes,
+triangle counts, layers, culling) and optionally in a JSON file.
 """
 import ctypes
 import json
@@ -17,6 +20,7 @@ import torch
 from diffren_amd import _native
 
 CANDIDATE_TILES = [(8, 8), (16, 16), (32, 8), (32, 16), (64, 8), (32, 32)]
+CANDIDATE_GROUPS = (1, 2, 4)
 HEADROOM = 1.25
 
 _cache = {}
@@ -101,3 +105,31 @@ def tune_raster(table_host, num_jobs, num_layers, culling, device,
   _cache[key] = dict(tile=[tw, th], capacity=capacity, timings=timings)
   _save(cache_path)
   return tw, th, capacity, timings
+
+
+def tune_groups(key, run_with, num_jobs, reps=40, warmup=5, cache_path=None,
+                candidates=None):
+  """Returns (groups, timings_ms): the number of job groups whose step runs
+  fastest.  run_with(n) prepares the renderer for n groups and returns a
+  function that enqueues one step (a graph replay); key identifies the
+  workload (signature() plus whatever else fixes the step)."""
+  _load(cache_path)
+  key = key + "/groups"
+  if key in _cache:
+    e = _cache[key]
+    return e["groups"], e["timings"]
+  timings = {}
+  best = None
+  for n in (candidates or CANDIDATE_GROUPS):
+    if n > num_jobs:
+      continue
+    fn = run_with(n)
+    for _ in range(warmup):
+      fn()
+    ms = time_launch(fn, reps)
+    timings[str(n)] = ms
+    if best is None or ms < best[0]:
+      best = (ms, n)
+  _cache[key] = dict(groups=best[1], timings=timings)
+  _save(cache_path)
+  return best[1], timings

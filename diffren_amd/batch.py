@@ -24,6 +24,7 @@ pixels by the fused kernels, and the shaded pixels' gradient is pushed back
 through the shader and the interpolation by autograd.
 """
 import ctypes
+import json
 
 import torch
 
@@ -50,7 +51,9 @@ class BatchedSplatRenderer:
            Cameras are copied at construction.
   tile:    raster tile (w, h), None for the default or "auto" for the
            autotuner (diffren_amd.autotune).
-  streams: number of job groups run concurrently (each on its own stream).
+  streams: number of job groups run concurrently (each on its own stream),
+           or "auto": the autotuner times the captured step with 1, 2 and 4
+           groups on this batch and keeps the fastest (streams_timings).
   fused_transform: when True no clip-space buffers are written: the
            rasterizer and the splat passes read the world-space vertices
            and transform them by the job's camera as they load them (same
@@ -231,25 +234,22 @@ class BatchedSplatRenderer:
     self.cameras = [j[1].to(torch.float32).contiguous() for j in jobs]
     self.autotune_cache = autotune_cache
     self.autotune_timings = None
-    self.groups = self._split(max(1, int(streams)))
-    # One loss slot per group, summed in a fixed order after the groups join.
-    self.loss_parts = torch.zeros(len(self.groups), dtype=torch.float32,
-                                  device=dev)
-    for k, g in enumerate(self.groups):
-      g.loss_slot = self.loss_parts[k:k + 1]
-    self.streams = [torch.cuda.Stream(device=dev) for _ in self.groups] \
-        if len(self.groups) > 1 else []
+    self.streams_timings = None
+    auto_groups = streams == "auto"
+    self._set_groups(1 if auto_groups else streams)
     if tile == "auto":
       tile, bin_capacity = self._autotune()
     # The fused passes read ids first, so background z / barycentrics are
     # never written (DR_RASTER_SPARSE).
     self.raster_config = rasterize._config(tile, bin_capacity,
                                            _native.RASTER_SPARSE)
+    self.graph = None
+    self._graph_forward_only = False
+    if auto_groups:
+      self._set_groups(self._autotune_groups())
     for g in self.groups:
       self._plan_raster(g)
       self._plan_splat(g)
-    self.graph = None
-    self._graph_forward_only = False
     # Bin-overflow watch.  Each group's rasterizer status words (entries
     # needed, overflow flag) are copied into pinned host memory at the end
     # of its raster stage, inside a captured graph too.  The next step() or
@@ -258,12 +258,50 @@ class BatchedSplatRenderer:
     # triangle record of their job: correct, slow), the bin capacity grows to
     # twice the entries needed, the workspaces are re-planned and a captured
     # graph is captured again.  `regrows` counts these.
-    self._status_host = torch.zeros((len(self.groups), 16), dtype=torch.uint8,
-                                    pin_memory=True)
     self._status_event = None
     self.regrows = 0
 
   # -- planning --------------------------------------------------------------
+
+  def _set_groups(self, n):
+    """Splits the jobs into n groups (each with its own stream and loss
+    slot); the groups still need _plan_raster / _plan_splat."""
+    dev = self.device
+    self.groups = self._split(max(1, int(n)))
+    # One loss slot per group, summed in a fixed order after the groups join.
+    self.loss_parts = torch.zeros(len(self.groups), dtype=torch.float32,
+                                  device=dev)
+    for k, g in enumerate(self.groups):
+      g.loss_slot = self.loss_parts[k:k + 1]
+    self.streams = [torch.cuda.Stream(device=dev) for _ in self.groups] \
+        if len(self.groups) > 1 else []
+    # Each group's raster status words (entries needed, overflow flag),
+    # copied to pinned host memory inside every step (see _check_overflow).
+    self._status_host = torch.zeros((len(self.groups), 16), dtype=torch.uint8,
+                                    pin_memory=True)
+
+  def _autotune_groups(self):
+    """Number of job groups from diffren_amd.autotune.tune_groups: the
+    captured step is timed for each candidate count on this batch."""
+    from diffren_amd import autotune
+
+    def run_with(n):
+      self._set_groups(n)
+      for g in self.groups:
+        self._plan_raster(g)
+        self._plan_splat(g)
+      return self.capture().replay
+
+    sizes = [(w, h, self.meshes[mi]["triangles"].shape[0])
+             for mi, _, w, h in self.jobs]
+    key = autotune.signature(sizes, self.L, 1) + json.dumps(
+        [self.raster_config.tile_w, self.raster_config.tile_h, self.C,
+         bool(self.fused_transform), bool(self.fixed_shift),
+         self.shading_function is not None])
+    n, self.streams_timings = autotune.tune_groups(
+        key, run_with, len(self.jobs), cache_path=self.autotune_cache)
+    self.graph = None
+    return n
 
   def _split(self, n):
     """Splits jobs into n groups of balanced pixel counts (greedy)."""

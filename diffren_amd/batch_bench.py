@@ -100,7 +100,8 @@ def run(args):
       "speedup_vs_per_view_fused": round(per_view_fused_ms / fused_ms, 2),
       "raster_tile": [r.raster_config.tile_w, r.raster_config.tile_h],
       "bin_capacity": int(r.raster_config.bin_capacity),
-      "streams": args.streams,
+      "streams": len(r.groups),
+      "streams_timings": r.streams_timings,
   }
 
 
@@ -110,7 +111,8 @@ def main():
   p.add_argument("--layers", type=int, default=3)
   p.add_argument("--steps", type=int, default=20)
   p.add_argument("--plain-steps", type=int, default=2)
-  p.add_argument("--streams", type=int, default=4)
+  p.add_argument("--streams", type=lambda v: v if v == "auto" else int(v),
+                 default="auto")
   p.add_argument("--seed", type=int, default=0)
   print(json.dumps(run(p.parse_args())), flush=True)
 

@@ -122,6 +122,40 @@ def test_graph_replays_match_eager(streams, device):
     torch.ones(1000, device=device).sum().item()
 
 
+def test_autotuned_group_count(device, tmp_path):
+  """streams="auto": the step is timed with 1, 2 and 4 job groups (as many
+  as there are jobs), the
+  fastest count is kept (and cached), and the step's results equal a
+  renderer built with that count."""
+  meshes = _meshes(device)
+  jobs = _jobs(meshes)
+  cache = str(tmp_path / "tune.json")
+  r = batch.BatchedSplatRenderer(meshes, jobs, 3, streams="auto",
+                                 autotune_cache=cache)
+  # Three jobs: one, two (four groups would exceed the job count).
+  assert set(r.streams_timings) == {"1", "2"}
+  best = min(r.streams_timings, key=r.streams_timings.get)
+  assert len(r.groups) == int(best)
+  f = batch.BatchedSplatRenderer(meshes, jobs, 3, streams=int(best))
+  for ta, tb in zip(r.targets, f.targets):
+    ta.uniform_()
+    tb.copy_(ta)
+  r.step()
+  f.step()
+  r.capture()
+  r.replay()
+  torch.cuda.synchronize()
+  for a, b in zip(r.images + r.d_vertices + r.d_colors,
+                  f.images + f.d_vertices + f.d_colors):
+    assert helpers.rel_err(a.cpu(), b.cpu()) <= 1e-5
+  assert abs(r.loss.item() - f.loss.item()) <= 1e-6 * abs(f.loss.item())
+  # A second renderer of the same batch takes the cached choice.
+  again = batch.BatchedSplatRenderer(meshes, jobs, 3, streams="auto",
+                                     autotune_cache=cache)
+  assert len(again.groups) == len(r.groups)
+  assert again.streams_timings == r.streams_timings
+
+
 @pytest.mark.parametrize("streams", [1, 4])
 def test_fused_transform_matches_clip_buffers(streams, device):
   """Vertex transform on load (no clip-space buffers) vs the transform
