@@ -172,8 +172,9 @@ class BatchedSplatRenderer:
     self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
     self.ids, self.z, self.barys = [], [], []
     self.images, self.targets, self.scratch = [], [], []
-    # Clip-space vertices: one [views, V, 4] block per mesh, one transform
-    # launch per mesh.
+    # Clip-space vertices: one [views, V, 4] block per mesh, its views laid
+    # out group by group (_layout_views), so each job group transforms its
+    # own views with one launch per mesh, on its own stream.
     self.mesh_views = {}
     for i, (mi, _, _, _) in enumerate(jobs):
       self.mesh_views.setdefault(mi, []).append(i)
@@ -279,6 +280,29 @@ class BatchedSplatRenderer:
     # copied to pinned host memory inside every step (see _check_overflow).
     self._status_host = torch.zeros((len(self.groups), 16), dtype=torch.uint8,
                                     pin_memory=True)
+    self._layout_views()
+
+  def _layout_views(self):
+    """Orders each mesh's views in its clip block group by group and gives
+    every group its (mesh, first view, view count) transform ranges."""
+    for g in self.groups:
+      g.transforms = []
+    if self.fused_transform:
+      return
+    for mi in self.mesh_views:
+      order = [i for g in self.groups for i in g.indices
+               if self.jobs[i][0] == mi]
+      self.mesh_views[mi] = order
+      self.camera_blocks[mi] = torch.stack(
+          [self.jobs[i][1].to(torch.float32).reshape(4, 4) for i in order])
+      for k, i in enumerate(order):
+        self.clip[i] = self.clip_blocks[mi][k]
+      start = 0
+      for g in self.groups:
+        n = sum(1 for i in g.indices if self.jobs[i][0] == mi)
+        if n:
+          g.transforms.append((mi, start, n))
+        start += n
 
   def _autotune_groups(self):
     """Number of job groups from diffren_amd.autotune.tune_groups: the
@@ -415,17 +439,21 @@ class BatchedSplatRenderer:
   # -- stages ----------------------------------------------------------------
 
   def transform(self):
+    """Clip-space vertices of every job (one launch per mesh)."""
     if self.fused_transform:
       return
-    lib = _native.lib()
-    s = _native.stream_handle()
     for mi, views in self.mesh_views.items():
-      v = self.meshes[mi]["vertices"]
-      _native.check(
-          lib.dr_transform_homogeneous_multi(
-              _native.ptr(self.camera_blocks[mi]), len(views), _native.ptr(v),
-              v.shape[1], v.shape[0], _native.ptr(self.clip_blocks[mi]), s),
-          "dr_transform_homogeneous_multi")
+      self._transform_views(mi, 0, len(views))
+
+  def _transform_views(self, mi, start, count):
+    """Views start .. start + count - 1 of mesh mi's clip block."""
+    v = self.meshes[mi]["vertices"]
+    _native.check(
+        _native.lib().dr_transform_homogeneous_multi(
+            _native.ptr(self.camera_blocks[mi][start]), count, _native.ptr(v),
+            v.shape[1], v.shape[0], _native.ptr(self.clip_blocks[mi][start]),
+            _native.stream_handle()),
+        "dr_transform_homogeneous_multi")
 
   def _rasterize(self, g, half=0):
     """half: 0 for the whole rasterizer, or _native.RASTER_BINS_ONLY /
@@ -503,6 +531,8 @@ class BatchedSplatRenderer:
     self.loss.zero_()
 
   def _group_raster(self, g):
+    for mi, start, count in g.transforms:
+      self._transform_views(mi, start, count)
     self._rasterize(g)
     self._status_host[self.groups.index(g)].copy_(g.raster_ws[:16],
                                                   non_blocking=True)
@@ -621,7 +651,6 @@ class BatchedSplatRenderer:
 
   def _enqueue_step(self):
     self.zero_grads()
-    self.transform()
     self._run_groups(1)
 
   def render(self):
@@ -631,7 +660,6 @@ class BatchedSplatRenderer:
     self._mark_step()
 
   def _enqueue_render(self):
-    self.transform()
     self._run_groups(0)
 
   # -- graphs ----------------------------------------------------------------
