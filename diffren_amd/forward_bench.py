@@ -108,7 +108,7 @@ def run(args):
           "algorithmic_bytes_per_view": per_view,
           "achieved_gbs": round(achieved, 1),
           "frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 4)},
-      "streams": args.streams,
+      "streams": len(r.groups),
   }
 
 
@@ -121,7 +121,8 @@ def main():
   p.add_argument("--layers", type=int, default=3)
   p.add_argument("--steps", type=int, default=20)
   p.add_argument("--plain-steps", type=int, default=3)
-  p.add_argument("--streams", type=int, default=4)
+  p.add_argument("--streams", type=lambda v: v if v == "auto" else int(v),
+                 default="auto")
   print(json.dumps(run(p.parse_args())), flush=True)
 
 
