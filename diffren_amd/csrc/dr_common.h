@@ -87,12 +87,40 @@ __device__ __forceinline__ T load_uniform(const T* p) {
   return out;
 }
 
+// Stores through global-address-space pointers: a generic pointer (read out
+// of a job table) would make them flat stores, which also count against the
+// LDS wait counter, so every later LDS wait or barrier would wait for them.
+template <typename T>
+__device__ __forceinline__ void gst(T* p, T v) {
+  *(__attribute__((address_space(1))) T*)(p) = v;
+}
+// 16-byte stores at p (16-byte aligned).
+__device__ __forceinline__ void gst4(float* p, float4 v) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  f4 q;
+  q.x = v.x; q.y = v.y; q.z = v.z; q.w = v.w;
+  *(__attribute__((address_space(1))) f4*)(p) = q;
+}
+__device__ __forceinline__ void gstu4(uint32_t* p, uint4 v) {
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+  u4 q;
+  q.x = v.x; q.y = v.y; q.z = v.z; q.w = v.w;
+  *(__attribute__((address_space(1))) u4*)(p) = q;
+}
+
 // Float atomic add through a global-address-space pointer (hardware
 // global_atomic_add_f32, relaxed, device scope; a generic pointer would make
 // it a flat atomic, which also counts against the LDS wait counter).
 __device__ __forceinline__ void gatomic_add(float* p, float v) {
   __hip_atomic_fetch_add((__attribute__((address_space(1))) float*)(p), v,
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// 64-bit integer atomic add through a global-address-space pointer.
+__device__ __forceinline__ void gatomic_add_u64(unsigned long long* p,
+                                                unsigned long long v) {
+  __hip_atomic_fetch_add(
+      (__attribute__((address_space(1))) unsigned long long*)(p), v,
+      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ inline int lane_id() { return __lane_id(); }
 

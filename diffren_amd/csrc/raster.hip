@@ -326,7 +326,8 @@ __device__ __forceinline__ float4 transform_point(const float* __restrict__ M,
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const float* m = M + 4 * i;
-    rp[i] = ((m[0] * x + m[1] * y) + m[2] * z) + m[3] * 1.0f;
+    rp[i] = ((ldg(m) * x + ldg(m + 1) * y) + ldg(m + 2) * z) +
+            ldg(m + 3) * 1.0f;
   }
   return r;
 }
@@ -349,7 +350,7 @@ __device__ __forceinline__ bool load_triangle(const dr_raster_job& job,
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const float* p = job.vertices + 3 * int64_t(idx[k]);
-      v[k] = transform_point(job.camera, p[0], p[1], p[2]);
+      v[k] = transform_point(job.camera, ldg(p), ldg(p + 1), ldg(p + 2));
     }
   } else {
 #pragma unroll
@@ -982,13 +983,13 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
       const int64_t pix = int64_t(iy) * W + ix;
       for (int l = 0; l < L; ++l) {
         const int64_t o = pix + l * P;
-        job.triangle_ids[o] = -1;
+        gst(job.triangle_ids + o, -1);
         if (sparse) continue;
-        job.z_buffer[o] = 1.0f;
+        gst(job.z_buffer + o, 1.0f);
         if (job.barycentrics) {
-          job.barycentrics[3 * o + 0] = 0.0f;
-          job.barycentrics[3 * o + 1] = 0.0f;
-          job.barycentrics[3 * o + 2] = 0.0f;
+          gst(job.barycentrics + 3 * o + 0, 0.0f);
+          gst(job.barycentrics + 3 * o + 1, 0.0f);
+          gst(job.barycentrics + 3 * o + 2, 0.0f);
         }
       }
     }
@@ -1125,13 +1126,13 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
         edge_setup(v, cull, r);
         shade_pixel(ix, iy, sx2, sy2, r, v, f);
       }
-      job.triangle_ids[o] = id;
+      gst(job.triangle_ids + o, id);
       if (sparse && id < 0) continue;
-      job.z_buffer[o] = f.z;
+      gst(job.z_buffer + o, f.z);
       if (job.barycentrics) {
-        job.barycentrics[3 * o + 0] = f.b0;
-        job.barycentrics[3 * o + 1] = f.b1;
-        job.barycentrics[3 * o + 2] = f.b2;
+        gst(job.barycentrics + 3 * o + 0, f.b0);
+        gst(job.barycentrics + 3 * o + 1, f.b1);
+        gst(job.barycentrics + 3 * o + 2, f.b2);
       }
     }
   }

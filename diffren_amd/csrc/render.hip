@@ -161,13 +161,15 @@ __device__ __forceinline__ float4 clip_vertex(const dr_splat_job& J, int v) {
   if (J.clip_vertices)
     return ldg4(J.clip_vertices + 4 * int64_t(v));
   const float* p = J.positions + 3 * int64_t(v);
-  const float x = p[0], y = p[1], z = p[2];
+  const float x = ldg(p), y = ldg(p + 1), z = ldg(p + 2);
   const float* M = J.camera;
   float4 r;
-  r.x = ((M[0] * x + M[1] * y) + M[2] * z) + M[3] * 1.0f;
-  r.y = ((M[4] * x + M[5] * y) + M[6] * z) + M[7] * 1.0f;
+  r.x = ((ldg(M) * x + ldg(M + 1) * y) + ldg(M + 2) * z) + ldg(M + 3) * 1.0f;
+  r.y = ((ldg(M + 4) * x + ldg(M + 5) * y) + ldg(M + 6) * z) +
+        ldg(M + 7) * 1.0f;
   r.z = 0.0f;
-  r.w = ((M[12] * x + M[13] * y) + M[14] * z) + M[15] * 1.0f;
+  r.w = ((ldg(M + 12) * x + ldg(M + 13) * y) + ldg(M + 14) * z) +
+        ldg(M + 15) * 1.0f;
   return r;
 }
 
@@ -235,6 +237,38 @@ __device__ __forceinline__ float tap_weight(float coef, float ox, float oy,
 
 __device__ __forceinline__ float norm_scale(float w) {
   return 1.0f / (fmaxf(w - 1.0f, 0.0f) + 1.0f);
+}
+
+// Stores a C-channel row to global memory (16-byte stores when C % 4 == 0:
+// rows are then 16-byte aligned); v == nullptr stores zeros.
+template <int C>
+__device__ __forceinline__ void store_row(float* dst, const float* v) {
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    if constexpr (C % 4 == 0) {
+      if (c % 4 == 0)
+        gst4(dst + c, v ? make_float4(v[c], v[c + 1], v[c + 2], v[c + 3])
+                        : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    } else {
+      gst(dst + c, v ? v[c] : 0.0f);
+    }
+  }
+}
+
+// Loads a C-channel row from global memory (16-byte loads when C % 4 == 0).
+template <int C>
+__device__ __forceinline__ void load_row(const float* src, float* v) {
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    if constexpr (C % 4 == 0) {
+      if (c % 4 == 0) {
+        const float4 q = ldg4(src + c);
+        v[c] = q.x; v[c + 1] = q.y; v[c + 2] = q.z; v[c + 3] = q.w;
+      }
+    } else {
+      v[c] = ldg(src + c);
+    }
+  }
 }
 
 // LDS: for all layers, the region's triangle ids st[L][kRegion] and viewport
@@ -466,28 +500,30 @@ void target_kernel(
     if (kAccum && (J.accum || J.norm || J.accum_weights)) {
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        if (J.accum_weights) J.accum_weights[k * P + p] = acc[k][C];
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          if (J.accum) J.accum[(k * P + p) * C + c] = acc[k][c];
-          if (J.norm) J.norm[(k * P + p) * C + c] = n[k][c];
-        }
+        if (J.accum_weights) gst(J.accum_weights + k * P + p, acc[k][C]);
+        if (J.accum) store_row<C>(J.accum + (k * P + p) * C, acc[k]);
+        if (J.norm) store_row<C>(J.norm + (k * P + p) * C, n[k]);
       }
     }
-    float g[C];
+    float o[C], g[C];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
-      float o = n[2][c];
-      o = n[1][c] + (1.0f - a1) * o;
-      o = n[0][c] + (1.0f - a0) * o;
-      if (J.image) J.image[p * C + c] = o;
-      if (mode == 1) {
-        const float d = o - ldg(J.target + p * C + c);
+      o[c] = n[2][c];
+      o[c] = n[1][c] + (1.0f - a1) * o[c];
+      o[c] = n[0][c] + (1.0f - a0) * o[c];
+    }
+    if (J.image) store_row<C>(J.image + p * C, o);
+    if (mode == 1) {
+      float t[C];
+      load_row<C>(J.target + p * C, t);
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const float d = o[c] - t[c];
         partial += d * d;
         g[c] = (2.0f * d) * loss_scale;
-      } else if (mode == 2) {
-        g[c] = ldg(J.d_image + p * C + c);
       }
+    } else if (mode == 2) {
+      load_row<C>(J.d_image + p * C, g);
     }
     // A pixel that received no splat weight is never read by the source
     // pass (every foreground source adds a positive weight to each of its
@@ -525,11 +561,12 @@ void target_kernel(
         for (int c = C + 1; c < GS; ++c) rec[c] = 0.0f;
 #pragma unroll
         for (int c = 0; c < GS; c += 4)
-          *reinterpret_cast<float4*>(gp + c) =
-              make_float4(rec[c], rec[c + 1], rec[c + 2], rec[c + 3]);
+          gst4(gp + c, make_float4(rec[c], rec[c + 1], rec[c + 2],
+                                   rec[c + 3]));
       }
-      *reinterpret_cast<uint4*>(J.grad_scratch + p * Scratch<C>::kHot) =
-          make_uint4(sw[0], sw[1], sw[2], 0u);
+      gstu4(reinterpret_cast<uint32_t*>(J.grad_scratch +
+                                        p * Scratch<C>::kHot),
+            make_uint4(sw[0], sw[1], sw[2], 0u));
     }
   }
   if (grad_mode == 1) {
@@ -575,22 +612,7 @@ struct Record {
       kShaded ? 4 : ((4 + C) <= 8 ? 8 : ((4 + C) <= 16 ? 16 : 32));
 };
 
-// Stores a pixel-layer's C-channel row (16-byte stores when C % 4 == 0);
-// v == nullptr stores zeros.
-template <int C>
-__device__ __forceinline__ void store_row(float* dst, const float* v) {
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    if constexpr (C % 4 == 0) {
-      if (c % 4 == 0)
-        *reinterpret_cast<float4*>(dst + c) =
-            v ? make_float4(v[c], v[c + 1], v[c + 2], v[c + 3])
-              : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    } else {
-      dst[c] = v ? v[c] : 0.0f;
-    }
-  }
-}
+
 
 // Source pass.  Float atomics to the vertex gradient records are the cost of
 // this pass (they run at a fixed rate of grouped operations), so it issues as
@@ -664,8 +686,8 @@ __device__ __forceinline__ void grad_add(float* p, int64_t i, float v,
                                          int fixed_shift, float fixed_scale) {
   if (fixed_shift > 0) {
     const long long q = llrintf(v * fixed_scale);
-    atomicAdd(reinterpret_cast<unsigned long long*>(p) + i,
-              static_cast<unsigned long long>(q));
+    gatomic_add_u64(reinterpret_cast<unsigned long long*>(p) + i,
+                    static_cast<unsigned long long>(q));
   } else {
     gatomic_add(p + i, v);
   }
@@ -909,7 +931,8 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
         const float* M = J.camera;
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-          dp[j] = (M[j] * dcx + M[4 + j] * dcy) + M[12 + j] * dcw;
+          dp[j] = (ldg(M + j) * dcx + ldg(M + 4 + j) * dcy) +
+                  ldg(M + 12 + j) * dcw;
         dp[3] = 0.0f;
       } else {
         dp[0] = dcx; dp[1] = dcy; dp[2] = 0.0f; dp[3] = dcw;
@@ -1213,6 +1236,14 @@ int dr_splat_jobs_prepare(dr_splat_job* jobs, int32_t n, int32_t channels,
                  "clip_vertices must be 16-byte aligned");
     DR_CHECK_ARG((reinterpret_cast<uintptr_t>(j.grad_scratch) & 15) == 0,
                  "grad_scratch must be 16-byte aligned");
+    const auto a16 = [](const void* q) {
+      return (reinterpret_cast<uintptr_t>(q) & 15) == 0;
+    };
+    DR_CHECK_ARG(channels % 4 != 0 ||
+                     (a16(j.image) && a16(j.target) && a16(j.d_image) &&
+                      a16(j.accum) && a16(j.norm)),
+                 "image, target, d_image, accum and norm must be 16-byte "
+                 "aligned when the channel count is a multiple of 4");
     DR_CHECK_ARG(j.d_camera == nullptr || (j.camera && j.positions),
                  "d_camera needs camera and positions");
     DR_CHECK_ARG(j.fixed_shift >= 0 && j.fixed_shift <= 62,

@@ -318,7 +318,9 @@ int dr_splat_bwd(const float* xyz, const float* color, const float* mask,
  * with shading_function returning the interpolated colours), or the shaded
  * pixels of any shading function (`shaded`, splat.py:178-229's
  * shaded_pixels).  All pointers are device pointers; C = channels, the last
- * channel is alpha. */
+ * channel is alpha.  When C is a multiple of 4 the [.., C] frames (image,
+ * target, d_image, accum, norm, shaded, d_shaded) must be 16-byte aligned:
+ * their rows move as 16-byte loads and stores. */
 typedef struct dr_splat_job {
   const float* clip_vertices;   /* [V, 4] clip space, 16-byte aligned; or
                                    NULL to transform `positions` by `camera`
