@@ -312,23 +312,30 @@ void target_kernel(
       grad_mode == 1 ? (J.target && J.grad_scratch ? 1 : 0)
                      : (grad_mode == 2 && J.d_image && J.grad_scratch ? 2 : 0);
 
-  // Ids and z of every layer (z buffer read only under a triangle).
+  // Ids and z of every layer (z buffer read only under a triangle).  A
+  // pixel's layers are a prefix of fragments (the rasterizer fills them
+  // front to back), so layer l's id is read only where layer l - 1 holds a
+  // fragment: background pixels cost one load.
   if (tid < kMaxLayers) layer_live[tid] = 0;
   __syncthreads();
-  for (int i = tid; i < L * R; i += kThreads) {
-    const int l = i / R, r = i % R;
+  for (int r = tid; r < R; r += kThreads) {
     const int px = x0 - 1 + r % kRX, py = y0 - 1 + r / kRX;
-    int t = -1;
-    float z = 1.0f;
-    if (px >= 0 && px < W && py >= 0 && py < H) {
-      const int64_t q = l * P + int64_t(py) * W + px;
-      t = ldg(J.triangle_ids + q);
-      if (t >= 0 && t < J.num_triangles) z = viewport_z(ldg(J.z_buffer + q));
-      else t = -1;
+    bool more = px >= 0 && px < W && py >= 0 && py < H;
+    const int64_t pix = int64_t(py) * W + px;
+    for (int l = 0; l < L; ++l) {
+      int t = -1;
+      float z = 1.0f;
+      if (more) {
+        const int64_t q = l * P + pix;
+        t = ldg(J.triangle_ids + q);
+        if (t >= 0 && t < J.num_triangles) z = viewport_z(ldg(J.z_buffer + q));
+        else t = -1;
+        more = t >= 0;
+      }
+      st[l * R + r] = t;
+      sz[l * R + r] = z;
+      if (t >= 0) layer_live[l] = 1;
     }
-    st[i] = t;
-    sz[i] = z;
-    if (t >= 0) layer_live[l] = 1;
   }
   __syncthreads();
 
@@ -779,9 +786,8 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
   // (most waves of a frame): it leaves before staging its patch, whose
   // records it would fetch from memory for nothing.  Waves share no LDS and
   // no block barrier, so a wave-uniform exit is safe.
-  bool any = false;
-  if (inside)
-    for (int l = 0; l < L; ++l) any |= ldg(J.triangle_ids + l * P + p) >= 0;
+  // Layers are a prefix of fragments, so layer 0 tells.
+  const bool any = inside && ldg(J.triangle_ids + p) >= 0;
   if (__ballot(any) == 0) {
     // d shaded is 0 wherever no fragment is (the fg mask zeroes the taps).
     if (d_shaded && inside)
