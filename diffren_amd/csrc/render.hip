@@ -848,10 +848,16 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
       surf[t >> 2] |= ((word >> (8 * (t & 3))) & 0xffu) << (8 * (t & 3));
     }
   }
-  // Every lane of the wave runs every layer iteration (ballots below).
-  for (int l = 0; l < L; ++l) {
+  // Every lane of the wave runs every layer iteration it visits (ballots
+  // below); the wave stops at its first layer without a fragment, since the
+  // deeper ones are empty too.
+  bool prev = any;  // this lane had a fragment in the previous layer
+  int l = 0;
+  for (; l < L; ++l) {
     Corner k = k0;
-    const bool frag = l == 0 ? frag0 : any && corners(J, l * P + p, k);
+    const bool frag = l == 0 ? frag0 : prev && corners(J, l * P + p, k);
+    prev = frag;
+    if (l > 0 && __ballot(frag) == 0) break;
     float base[NB];
     float dclip[3] = {0.0f, 0.0f, 0.0f};  // d clip x, y, w
     float dcol[C];
@@ -1103,6 +1109,9 @@ __global__ __launch_bounds__(kBX* kBY) void source_kernel(
     }
     wave_sync();
   }
+  // The layers the wave did not visit hold no fragment: d shaded is 0.
+  if (d_shaded && inside)
+    for (; l < L; ++l) store_row<C>(d_shaded + (l * P + p) * C, nullptr);
 }
 
 __global__ void fixed_to_float_kernel(const int64_t* __restrict__ fixed,
