@@ -181,10 +181,11 @@ def pmc_probe(args):
   w = build_workload(args, dev)
   R, L = args.resolution, args.layers
   jobs = [(0, w["cams_t"][i], R, R) for i in range(args.views)]
-  # Four fixed groups of two views when not given: the counters' per-launch
-  # figures then cover two views, whatever the autotuner would pick.
+  # Two fixed groups of four views when not given (the count the autotuner
+  # picks for this workload): the counters' per-launch figures then cover
+  # the bench's launches.
   r = batch.BatchedSplatRenderer([w["mesh"]], jobs, L, tile=(16, 16),
-                                 streams=(4 if args.streams == "auto" else
+                                 streams=(2 if args.streams == "auto" else
                                           args.streams),
                                  fused_transform=args.fused_transform)
   for t in r.targets:
