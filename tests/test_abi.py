@@ -115,6 +115,15 @@ def test_render_channel_counts_and_job_checks():
   assert lib.dr_splat_jobs_prepare(job, 1, 4, ctypes.byref(tiles)) != 0
   assert b"grad_scratch" in lib.dr_last_error()
   job[0].grad_scratch = 0
+  # Frames of C % 4 == 0 channels move as 16-byte rows: a misaligned image
+  # is refused there, and taken for other channel counts.
+  job[0].image = 264
+  assert lib.dr_splat_jobs_prepare(job, 1, 4, ctypes.byref(tiles)) != 0
+  assert b"16-byte" in lib.dr_last_error()
+  assert lib.dr_splat_jobs_prepare(job, 1, 3, ctypes.byref(tiles)) == 0
+  job[0].image = 272
+  assert lib.dr_splat_jobs_prepare(job, 1, 4, ctypes.byref(tiles)) == 0
+  job[0].image = 0
   job[0].num_triangles = 3
   assert lib.dr_splat_jobs_prepare(job, 1, 4, ctypes.byref(tiles)) != 0
   assert b"null mesh" in lib.dr_last_error()
