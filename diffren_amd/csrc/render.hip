@@ -459,10 +459,12 @@ void target_kernel(
       for (int t = 0; t < 9; ++t) {
         const int u = t / 3 - 1, v = t % 3 - 1;
         const int i = rT - v * kRX - u;
+        // Background or outside the image (x NaN): weight 0 -- a select,
+        // not a branch (its colour row is 0, so the adds change nothing).
         const float px = sx[i];
-        if (px != px) continue;  // background or outside the image
         float dx, dy;
-        const float w = tap_weight(coef, px, sy[i], u, v, dx, dy);
+        const float w0 = tap_weight(coef, px, sy[i], u, v, dx, dy);
+        const float w = px == px ? w0 : 0.0f;
         const int kc = layer_class(
             int(int8_t(sw[t >> 2] >> (8 * (t & 3)))), l);
         float cv[C];
