@@ -610,11 +610,13 @@ __global__ __launch_bounds__(kBlock) void bin_fill_kernel(
     if (has) tile = entry_tile<TW, TH>(e, k, pix);
     int leader, rank, size;
     group_by_tile(has, tile, leader, rank, size);
+    // The tile's bin offset is read before the cursor atomic returns.
+    const uint32_t tile_off = has ? ldg(offsets + tile) : 0u;
     uint32_t base = 0;
     if (has && rank == 0) base = atomicAdd(&counter[tile], uint32_t(size));
     base = __shfl(base, leader, 64);
     if (has) {
-      const uint64_t pos = uint64_t(offsets[tile]) + base + uint32_t(rank);
+      const uint64_t pos = uint64_t(tile_off) + base + uint32_t(rank);
       if (pos < capacity) {
         uint32_t hi = kTriMarker;
         if (e.w0 & kHeadFrag) {
