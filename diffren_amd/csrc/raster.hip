@@ -247,28 +247,79 @@ struct Fragment {
   float b0, b1, b2, z;
 };
 
+// Edge value r0 * px + r1 * py + r2 * 2^16 of one adjugate row at the
+// fixed-point pixel centre (px, py), in wrapping int64 arithmetic.
+__device__ __forceinline__ int64_t edge_at(const int64_t r[3], int32_t px,
+                                           int32_t py) {
+  return wadd(wadd(wmul(r[0], px), wmul(r[1], py)),
+              static_cast<int64_t>(static_cast<uint64_t>(r[2]) << 16));
+}
+__device__ __forceinline__ int64_t edge_sum(int64_t a, int64_t b, int64_t c) {
+  return wadd(wadd(a, b), c);
+}
+
+// The same edge values in double precision, from the row (r0, r1, r2 * 2^16)
+// as doubles.  When every adjugate entry is below 2^33 in magnitude (true of
+// every triangle a few pixels across), each product and partial sum below is
+// an integer under 2^51, so the two FMAs are exact and give the int64 value
+// (which then does not wrap), and so is the sum of three edges.  A 64-bit
+// integer product takes several quarter-rate multiplies; a double FMA is one
+// full-rate instruction, and its conversion to float one more.  Neither FMA
+// returns -0 (the constant term is an integer converted to double, never -0),
+// so a zero edge converts to +0 as in the integer path.
+__device__ __forceinline__ double edge_at(const double r[3], int32_t px,
+                                          int32_t py) {
+  return fma(r[0], static_cast<double>(px),
+             fma(r[1], static_cast<double>(py), r[2]));
+}
+__device__ __forceinline__ double edge_sum(double a, double b, double c) {
+  return (a + b) + c;
+}
+
+// Double rows of the adjugate r for edge_at; false (and rows unused) when
+// some entry is not below 2^33 in magnitude.
+__device__ __forceinline__ bool edges_f64(const int64_t r[9], double d[9]) {
+  bool exact = true;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    exact &= static_cast<uint64_t>(r[i]) + (uint64_t(1) << 33) <
+             (uint64_t(1) << 34);
+    d[i] = static_cast<double>(r[i]) * (i % 3 == 2 ? 65536.0 : 1.0);
+  }
+  return exact;
+}
+
+// Calls fn(rows) with the adjugate's exact double rows when edges_f64
+// accepts them (every triangle a few pixels across), else with the int64
+// adjugate; shade_pixel gives the same bits with either.
+template <typename Fn>
+__device__ __forceinline__ void with_edges(const int64_t r[9], Fn&& fn) {
+  double d[9];
+  if (edges_f64(r, d)) fn(d); else fn(r);
+}
+
 // Edge test at pixel (ix, iy); rasterize_utils.h:220-248 and
 // rasterize_triangles_impl.cc:73-92.  Returns true when the pixel is inside
-// and its z is storable.
+// and its z is storable.  `r` holds the adjugate as int64 or, when
+// edges_f64 accepted it, its exact double rows; both give the same bits.
+template <typename E>
 __device__ __forceinline__ bool shade_pixel(int ix, int iy, float sx2,
-                                            float sy2, const int64_t r[9],
+                                            float sy2, const E r[9],
                                             const float4 v[3], Fragment& f) {
   const float px = sx2 * (static_cast<float>(ix) + 0.5f) - 1.0f;
   const float py = sy2 * (static_cast<float>(iy) + 0.5f) - 1.0f;
   // Pixel centres of the image lie in (-1, 1), so px * 2^16 is within 32-bit
   // range and its 32-bit truncation is to_fixed's value exactly; the
   // sign-extended 32-bit operand makes the edge products cheaper.
-  const int64_t pxi = static_cast<int32_t>(px * 65536.0f);
-  const int64_t pyi = static_cast<int32_t>(py * 65536.0f);
-  int64_t e[3];
+  const int32_t pxi = static_cast<int32_t>(px * 65536.0f);
+  const int32_t pyi = static_cast<int32_t>(py * 65536.0f);
+  E e[3];
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
-    e[i] = wadd(wadd(wmul(r[3 * i], pxi), wmul(r[3 * i + 1], pyi)),
-                static_cast<int64_t>(static_cast<uint64_t>(r[3 * i + 2]) << 16));
+  for (int i = 0; i < 3; ++i) e[i] = edge_at(r + 3 * i, pxi, pyi);
   if (!((e[0] >= 0 && e[1] >= 0 && e[2] >= 0) &&
         (e[0] > 0 || e[1] > 0 || e[2] > 0)))
     return false;
-  const float sum = static_cast<float>(wadd(wadd(e[0], e[1]), e[2]));
+  const float sum = static_cast<float>(edge_sum(e[0], e[1], e[2]));
   f.b0 = static_cast<float>(e[0]) / sum;
   f.b1 = static_cast<float>(e[1]) / sum;
   f.b2 = static_cast<float>(e[2]) / sum;
@@ -497,6 +548,64 @@ __device__ __forceinline__ void group_by_tile(bool has, uint32_t tile,
   }
 }
 
+// Count-pass work of one loaded triangle.  Instantiated separately for
+// small_coords triangles, whose fixed-point entries are sign-extended 32-bit
+// values: their cull test and adjugate then use 32 x 32 -> 64-bit products
+// (a merged matrix would make every product a full 64-bit one).
+template <int TW, int TH, bool kSmall>
+__device__ __forceinline__ void count_triangle(const dr_raster_job& job,
+                                               const float4 v[3], int cull,
+                                               int64_t flat, LaneEntries& e,
+                                               uint4* __restrict__ frag_z) {
+  int64_t m[9];
+  if (kSmall) fixed_matrix_small(v, m); else fixed_matrix(v, m);
+  // The exact cull first: it needs only the fixed-point matrix, and the
+  // triangles it rejects (most back faces) then skip the clipped box and
+  // its six divisions.  Either test alone means no entries.
+  Box b;
+  if (cannot_cover(m, cull) || !triangle_bbox(v, job.width, job.height, b))
+    return;
+  if (b.right - b.left <= kSmallSide && b.top - b.bottom <= kSmallSide) {
+    // The tile pass's arithmetic exactly (edge_setup's adjugate of the
+    // same fixed-point matrix, shade_pixel).
+    int64_t r[9];
+    adjugate(m, cull, r);
+    const float sx2 = 2.0f / static_cast<float>(job.width);
+    const float sy2 = 2.0f / static_cast<float>(job.height);
+    int nf = 0;
+    uint32_t offs = 0;
+    uint32_t z[kInlineFrags] = {0u, 0u, 0u, 0u, 0u, 0u};
+    auto scan_box = [&](const auto* edges) {
+      for (int iy = b.bottom; iy < b.top; ++iy)
+        for (int ix = b.left; ix < b.right; ++ix) {
+          Fragment f;
+          if (!shade_pixel(ix, iy, sx2, sy2, edges, v, f)) continue;
+          if (nf < kInlineFrags) {
+            offs |= uint32_t(((iy - b.bottom) << 2) | (ix - b.left))
+                    << (4 * nf);
+#pragma unroll
+            for (int k = 0; k < kInlineFrags; ++k)
+              if (k == nf) z[k] = __float_as_uint(f.z);
+          }
+          ++nf;
+        }
+    };
+    double d[9];
+    if (edges_f64(r, d)) scan_box(d); else scan_box(r);
+    if (nf <= kInlineFrags) {
+      e.n = nf;
+      if (nf > 0) {
+        e.w0 = kHeadFrag | uint32_t(nf) | (offs << 3);
+        e.w1 = uint32_t(b.left) | (uint32_t(b.bottom) << 16);
+        frag_z[2 * flat] = make_uint4(z[0], z[1], z[2], z[3]);
+        if (nf > 4) frag_z[2 * flat + 1] = make_uint4(z[4], z[5], 0u, 0u);
+      }
+      return;
+    }
+  }
+  whole_triangle<TW, TH>(job, b, e);
+}
+
 template <int TW, int TH>
 __global__ __launch_bounds__(kBlock) void bin_count_kernel(
     Jobs J, int64_t total_tris, int cull, uint32_t* __restrict__ counter,
@@ -509,54 +618,11 @@ __global__ __launch_bounds__(kBlock) void bin_count_kernel(
     e.tile_begin = job.tile_begin;
     e.ntx = tiles_x(job, TW);
     float4 v[3];
-    Box b;
-    int64_t m[9];
-    // The exact cull first: it needs only the fixed-point matrix, and the
-    // triangles it rejects (most back faces) then skip the clipped box and
-    // its six divisions.  Either test alone means no entries.
-    bool live = load_triangle(job, e.t, v);
-    if (live) {
-      if (small_coords(v)) fixed_matrix_small(v, m); else fixed_matrix(v, m);
-      live = !cannot_cover(m, cull) &&
-             triangle_bbox(v, job.width, job.height, b);
-    }
-    if (live) {
-      bool whole = true;
-      if (b.right - b.left <= kSmallSide && b.top - b.bottom <= kSmallSide) {
-        // The tile pass's arithmetic exactly (edge_setup's adjugate of the
-        // same fixed-point matrix, shade_pixel).
-        int64_t r[9];
-        adjugate(m, cull, r);
-        const float sx2 = 2.0f / static_cast<float>(job.width);
-        const float sy2 = 2.0f / static_cast<float>(job.height);
-        int nf = 0;
-        uint32_t offs = 0;
-        uint32_t z[kInlineFrags] = {0u, 0u, 0u, 0u, 0u, 0u};
-        for (int iy = b.bottom; iy < b.top; ++iy)
-          for (int ix = b.left; ix < b.right; ++ix) {
-            Fragment f;
-            if (!shade_pixel(ix, iy, sx2, sy2, r, v, f)) continue;
-            if (nf < kInlineFrags) {
-              offs |= uint32_t(((iy - b.bottom) << 2) | (ix - b.left))
-                      << (4 * nf);
-#pragma unroll
-              for (int k = 0; k < kInlineFrags; ++k)
-                if (k == nf) z[k] = __float_as_uint(f.z);
-            }
-            ++nf;
-          }
-        if (nf <= kInlineFrags) {
-          whole = false;
-          e.n = nf;
-          if (nf > 0) {
-            e.w0 = kHeadFrag | uint32_t(nf) | (offs << 3);
-            e.w1 = uint32_t(b.left) | (uint32_t(b.bottom) << 16);
-            frag_z[2 * flat] = make_uint4(z[0], z[1], z[2], z[3]);
-            if (nf > 4) frag_z[2 * flat + 1] = make_uint4(z[4], z[5], 0u, 0u);
-          }
-        }
-      }
-      if (whole) whole_triangle<TW, TH>(job, b, e);
+    if (load_triangle(job, e.t, v)) {
+      if (small_coords(v))
+        count_triangle<TW, TH, true>(job, v, cull, flat, e, frag_z);
+      else
+        count_triangle<TW, TH, false>(job, v, cull, flat, e, frag_z);
     }
     head[flat] = make_uint2(e.w0, e.w1);
   }
@@ -777,7 +843,11 @@ __device__ __forceinline__ bool fragment_at(const dr_raster_job& job, int t,
     return false;
   int64_t r[9];
   edge_setup(v, cull, r);
-  return shade_pixel(ix, iy, sx2, sy2, r, v, f);
+  bool hit = false;
+  with_edges(r, [&](const auto* e) {
+    hit = shade_pixel(ix, iy, sx2, sy2, e, v, f);
+  });
+  return hit;
 }
 
 // Reference insertion of (z, id) into the L-deep list lz/lid.
@@ -1054,13 +1124,15 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
         } else if (area > 0) {
           int64_t r[9];
           edge_setup(v, cull, r);
-          for (int iy = rect.y0; iy < rect.y1; ++iy)
-            for (int ix = rect.x0; ix < rect.x1; ++ix) {
-              Fragment f;
-              if (shade_pixel(ix, iy, sx2, sy2, r, v, f))
-                insert_key(keys, ties, TP, (iy - y0) * TW + (ix - x0), L,
-                           (uint64_t(z_order(f.z)) << 32) | uint32_t(t));
-            }
+          with_edges(r, [&](const auto* e) {
+            for (int iy = rect.y0; iy < rect.y1; ++iy)
+              for (int ix = rect.x0; ix < rect.x1; ++ix) {
+                Fragment f;
+                if (shade_pixel(ix, iy, sx2, sy2, e, v, f))
+                  insert_key(keys, ties, TP, (iy - y0) * TW + (ix - x0), L,
+                             (uint64_t(z_order(f.z)) << 32) | uint32_t(t));
+              }
+          });
         }
       }
     }
@@ -1084,13 +1156,15 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
       edge_setup(v, cull, r);
       const int rw = rq.x1 - rq.x0;
       const int area = rw * (rq.y1 - rq.y0);
-      for (int k = lane_id(); k < area; k += 64) {
-        const int ix = rq.x0 + k % rw, iy = rq.y0 + k / rw;
-        Fragment f;
-        if (shade_pixel(ix, iy, sx2, sy2, r, v, f))
-          insert_key(keys, ties, TP, (iy - y0) * TW + (ix - x0), L,
-                     (uint64_t(z_order(f.z)) << 32) | uint32_t(rq.tri));
-      }
+      with_edges(r, [&](const auto* e) {
+        for (int k = lane_id(); k < area; k += 64) {
+          const int ix = rq.x0 + k % rw, iy = rq.y0 + k / rw;
+          Fragment f;
+          if (shade_pixel(ix, iy, sx2, sy2, e, v, f))
+            insert_key(keys, ties, TP, (iy - y0) * TW + (ix - x0), L,
+                       (uint64_t(z_order(f.z)) << 32) | uint32_t(rq.tri));
+        }
+      });
     }
     __syncthreads();
   }
@@ -1126,7 +1200,9 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
         load_triangle(job, id, v);
         int64_t r[9];
         edge_setup(v, cull, r);
-        shade_pixel(ix, iy, sx2, sy2, r, v, f);
+        with_edges(r, [&](const auto* e) {
+          shade_pixel(ix, iy, sx2, sy2, e, v, f);
+        });
       }
       gst(job.triangle_ids + o, id);
       if (sparse && id < 0) continue;
