@@ -211,6 +211,53 @@ def test_deep_layer_stacks(layers, tile, device):
   assert (got[0][min(layers, 40) - 1] >= 0).any()
 
 
+def _scaled_scene(seed, n, width):
+  """`n` triangles of 0.5-12 pixels at random places and depths, each with
+  its clip coordinates scaled by w = 2^k, k in [-3, 16], half of them wound
+  clockwise.  Scaling by w leaves the screen triangle but multiplies the
+  fixed-point matrix by w and its adjugate by w^2, so covering triangles
+  fall on both sides of 2^33 (where the kernels switch between exact double
+  and int64 edge values), and those with coordinates of 2^15 and more take
+  the general fixed-point path (64-bit entries, wrapping products)."""
+  rng = np.random.default_rng(seed)
+  v = []
+  for _ in range(n):
+    c = rng.uniform(-1.0, 1.0, 2)
+    size = rng.uniform(0.5, 12.0) * 2.0 / width
+    w = 2.0 ** int(rng.integers(-3, 17))
+    z = rng.uniform(-0.9, 0.9)
+    for k in range(3):
+      a = 2 * np.pi * (k / 3.0 + rng.uniform(-0.12, 0.12))
+      p = c + size * rng.uniform(0.5, 1.0) * np.array([np.cos(a), np.sin(a)])
+      v.append([p[0] * w, p[1] * w, (z + rng.uniform(-0.05, 0.05)) * w, w])
+  t = np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+  t[::2] = t[::2][:, [0, 2, 1]]
+  return np.array(v, np.float32), t
+
+
+def _adjugate_max(v, t):
+  """Largest adjugate entry magnitude of each triangle's fixed-point matrix
+  (columns (x, y, w) * 2^16 of its vertices; the adjugate's rows are the
+  columns' cross products), in float64 -- enough to tell 2^32 from 2^34."""
+  m = np.trunc(v[t][:, :, [0, 1, 3]].astype(np.float64) * 65536.0)
+  r = [np.cross(m[:, 1], m[:, 2]), np.cross(m[:, 2], m[:, 0]),
+       np.cross(m[:, 0], m[:, 1])]
+  return np.abs(np.concatenate(r, 1)).max(1)
+
+
+@pytest.mark.parametrize("cull", [0, 1, 2])
+def test_edge_value_paths_bit_exact(cull, device):
+  # Triangles whose adjugates lie below and above 2^33, in the count pass's
+  # inline path (boxes of at most 4 x 4 pixels) and the tile pass's.
+  v, t = _scaled_scene(40 + cull, 600, 96)
+  want = raster_oracle.rasterize(v, t, 96, 80, 3, cull)
+  assert_same(gpu_raster(v, t, 96, 80, 3, cull, device), want)
+  hit = np.unique(want[0][want[0] >= 0])
+  rmax = _adjugate_max(v, t)[hit]
+  assert (rmax < 2.0 ** 32).sum() > 20 and (rmax > 2.0 ** 34).sum() > 20
+  assert (np.abs(v[t[hit]]).max((1, 2)) >= 32768).any()
+
+
 @pytest.mark.parametrize("w,h", [(1, 1), (2048, 160), (160, 2048), (31, 1)])
 def test_extreme_image_shapes(w, h, device):
   v, t = golden_scenes.adversarial_scene(900 + w, 50, 200)
