@@ -21,6 +21,7 @@ _c_size = ctypes.c_size_t
 RASTER_SPARSE = 1
 RASTER_BINS_ONLY = 2
 RASTER_TILES_ONLY = 4
+RASTER_PREFIX = 8
 RENDER_ACCUM_BUFFERS = 4
 RENDER_SHADED = 8
 MLP_ALBEDO_MODULATE = 1

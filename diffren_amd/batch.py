@@ -170,7 +170,7 @@ class BatchedSplatRenderer:
     self.records_fixed = ([torch.zeros(r.shape, dtype=torch.int64, device=dev)
                            for r in self.records] if deterministic else [])
     self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
-    self.ids, self.z, self.barys = [], [], []
+    self._ids, self.z, self.barys = [], [], []
     self.images, self.targets, self.scratch = [], [], []
     # Clip-space vertices: one [views, V, 4] block per mesh, its views laid
     # out group by group (_layout_views), so each job group transforms its
@@ -206,8 +206,8 @@ class BatchedSplatRenderer:
                                               device=dev))
     npix = 0
     for mi, cam, w, h in jobs:
-      self.ids.append(torch.empty((self.L, h, w), dtype=torch.int32,
-                                  device=dev))
+      self._ids.append(torch.empty((self.L, h, w), dtype=torch.int32,
+                                   device=dev))
       self.z.append(torch.empty((self.L, h, w), dtype=torch.float32,
                                 device=dev))
       self.barys.append(torch.empty((self.L, h, w, 3), dtype=torch.float32,
@@ -238,12 +238,17 @@ class BatchedSplatRenderer:
     self.streams_timings = None
     auto_groups = streams == "auto"
     self._set_groups(1 if auto_groups else streams)
+    # The fused passes read ids first, so background z / barycentrics are
+    # never written (DR_RASTER_SPARSE), and stop at a pixel's first empty
+    # layer, so the layers behind it are not written either
+    # (DR_RASTER_PREFIX; the ids property completes them for other readers).
+    # A shading function interpolates every layer: it gets complete ids.
+    self._raster_flags = _native.RASTER_SPARSE | (
+        _native.RASTER_PREFIX if shading_function is None else 0)
     if tile == "auto":
       tile, bin_capacity = self._autotune()
-    # The fused passes read ids first, so background z / barycentrics are
-    # never written (DR_RASTER_SPARSE).
     self.raster_config = rasterize._config(tile, bin_capacity,
-                                           _native.RASTER_SPARSE)
+                                           self._raster_flags)
     self.graph = None
     self._graph_forward_only = False
     if auto_groups:
@@ -351,7 +356,7 @@ class BatchedSplatRenderer:
       else:
         table[k].vertices = self.clip[i].data_ptr()
       table[k].triangles = m["triangles"].data_ptr()
-      table[k].triangle_ids = self.ids[i].data_ptr()
+      table[k].triangle_ids = self._ids[i].data_ptr()
       table[k].z_buffer = self.z[i].data_ptr()
       table[k].barycentrics = self.barys[i].data_ptr()
       table[k].num_vertices = m["vertices"].shape[0]
@@ -369,7 +374,7 @@ class BatchedSplatRenderer:
     tw, th, cap, timings = autotune.tune_raster(
         self._raster_table(list(range(len(self.jobs)))), len(self.jobs),
         self.L, 1, self.device, sizes, cache_path=self.autotune_cache,
-        flags=_native.RASTER_SPARSE)
+        flags=self._raster_flags)
     self.autotune_timings = timings
     return (tw, th), cap
 
@@ -409,7 +414,7 @@ class BatchedSplatRenderer:
       else:
         j.colors = m["colors"].data_ptr()
       j.camera = self.cameras[i].data_ptr()
-      j.triangle_ids = self.ids[i].data_ptr()
+      j.triangle_ids = self._ids[i].data_ptr()
       j.z_buffer = self.z[i].data_ptr()
       j.barycentrics = self.barys[i].data_ptr()
       j.image = self.images[i].data_ptr()
@@ -520,6 +525,18 @@ class BatchedSplatRenderer:
           _native.ptr(src), src.numel(), self.fixed_shift, _native.ptr(dst),
           _native.stream_handle()), "dr_fixed_to_float")
 
+  @property
+  def ids(self):
+    """Triangle ids [L, h, w] of each job (-1 on background).  The fused
+    raster writes a pixel's layers only up to its first empty one
+    (DR_RASTER_PREFIX); reading them here first sets the layers behind it
+    to -1, in place, on the current stream."""
+    if self._raster_flags & _native.RASTER_PREFIX:
+      for t in self._ids:
+        for l in range(1, self.L):
+          t[l].masked_fill_(t[l - 1] < 0, -1)
+    return self._ids
+
   def zero_grads(self):
     # Fill kernels, not memsets, so the step can be graph-captured.
     for t in self.records + self.records_fixed:
@@ -570,7 +587,7 @@ class BatchedSplatRenderer:
       for i, (mi, _, w, h) in enumerate(self.jobs):
         m = self.meshes[mi]
         fb = framebuffer.Framebuffer(
-            barycentrics=self.barys[i], triangle_id=self.ids[i].unsqueeze(-1),
+            barycentrics=self.barys[i], triangle_id=self._ids[i].unsqueeze(-1),
             triangles=m["triangles"], z_buffer=self.z[i])
         # Background 0 (render.py:124), written by the kernel itself (no
         # background tensor: nothing is copied from the host, so the step

@@ -1010,7 +1010,7 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
     const uint32_t* __restrict__ offsets,
     const uint64_t* __restrict__ bin_key, const uint16_t* __restrict__ bin_pix,
     const uint2* __restrict__ head, const uint32_t* __restrict__ frag_z,
-    uint64_t capacity, bool sparse) {
+    uint64_t capacity, bool sparse, bool prefix) {
   constexpr int TP = TW * TH;
   extern __shared__ uint64_t keys[];  // [L][TP]
   __shared__ TriRect queue[kBlock];
@@ -1056,13 +1056,15 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
       for (int l = 0; l < L; ++l) {
         const int64_t o = pix + l * P;
         gst(job.triangle_ids + o, -1);
-        if (sparse) continue;
-        gst(job.z_buffer + o, 1.0f);
-        if (job.barycentrics) {
-          gst(job.barycentrics + 3 * o + 0, 0.0f);
-          gst(job.barycentrics + 3 * o + 1, 0.0f);
-          gst(job.barycentrics + 3 * o + 2, 0.0f);
+        if (!sparse) {
+          gst(job.z_buffer + o, 1.0f);
+          if (job.barycentrics) {
+            gst(job.barycentrics + 3 * o + 0, 0.0f);
+            gst(job.barycentrics + 3 * o + 1, 0.0f);
+            gst(job.barycentrics + 3 * o + 2, 0.0f);
+          }
         }
+        if (prefix) break;
       }
     }
     return;
@@ -1205,13 +1207,16 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
         });
       }
       gst(job.triangle_ids + o, id);
-      if (sparse && id < 0) continue;
-      gst(job.z_buffer + o, f.z);
-      if (job.barycentrics) {
-        gst(job.barycentrics + 3 * o + 0, f.b0);
-        gst(job.barycentrics + 3 * o + 1, f.b1);
-        gst(job.barycentrics + 3 * o + 2, f.b2);
+      if (!sparse || id >= 0) {
+        gst(job.z_buffer + o, f.z);
+        if (job.barycentrics) {
+          gst(job.barycentrics + 3 * o + 0, f.b0);
+          gst(job.barycentrics + 3 * o + 1, f.b1);
+          gst(job.barycentrics + 3 * o + 2, f.b2);
+        }
       }
+      // The key lists fill front to back: no fragment behind an empty layer.
+      if (prefix && id < 0) break;
     }
   }
 }
@@ -1325,7 +1330,7 @@ int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
   raster_tile_kernel<TW, TH><<<unsigned(total_tiles), kBlock, lds, s>>>(
       J, L, cull, w.counts, w.offsets, w.bin_key, w.bin_pix, w.head,
       reinterpret_cast<const uint32_t*>(w.frag_z), capacity,
-      (flags & DR_RASTER_SPARSE) != 0);
+      (flags & DR_RASTER_SPARSE) != 0, (flags & DR_RASTER_PREFIX) != 0);
   DR_CHECK_LAUNCH("raster tiles");
   return kOk;
 }

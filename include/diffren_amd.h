@@ -96,6 +96,15 @@ typedef struct dr_raster_config {
  * render), it saves the background writes of every layer. */
 enum { DR_RASTER_SPARSE = 1 };
 
+/* A pixel's layers form a prefix: it holds a layer-l fragment only if it
+ * holds one at every layer before l.  With DR_RASTER_PREFIX the entries past
+ * a pixel's first empty layer are not written at all (that first empty
+ * layer is, with id -1): a consumer that stops at a pixel's first id < 0,
+ * as the fused render passes do, reads only written entries, and the deeper
+ * background layers' writes are saved.  Other readers must first set the
+ * ids past the first -1 to -1. */
+enum { DR_RASTER_PREFIX = 8 };
+
 /* A call normally bins the triangles into screen tiles and then rasterizes
  * the tiles.  DR_RASTER_BINS_ONLY runs the binning half only; a later call
  * with DR_RASTER_TILES_ONLY and the same jobs, workspace and config runs the

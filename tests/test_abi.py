@@ -59,6 +59,16 @@ def test_header_enums_match_python():
   text = open(os.path.join(ROOT, "include", "diffren_amd.h")).read()
   assert "DR_CULL_NONE = 0, DR_CULL_BACK = 1, DR_CULL_FRONT = 2" in text
   assert [int(m) for m in constants.FaceCullingMode] == [0, 1, 2]
+  # Raster flag bits: header values against the Python mirrors, and no two
+  # flags sharing a bit.
+  import re
+  flags = {n: int(v) for n, v in
+           re.findall(r"DR_RASTER_([A-Z_]+) = (\d+)", text)}
+  assert flags == {"SPARSE": _native.RASTER_SPARSE,
+                   "BINS_ONLY": _native.RASTER_BINS_ONLY,
+                   "TILES_ONLY": _native.RASTER_TILES_ONLY,
+                   "PREFIX": _native.RASTER_PREFIX}
+  assert sum(flags.values()) == (1 | 2 | 4 | 8)
 
 
 def test_struct_layouts_match_header(tmp_path):

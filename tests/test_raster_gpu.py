@@ -8,6 +8,7 @@ import torch
 
 import helpers
 import scenes as golden_scenes
+from diffren_amd import _native
 from diffren_amd import rasterize
 from diffren_amd import scenes
 from oracle import raster_oracle
@@ -256,6 +257,35 @@ def test_edge_value_paths_bit_exact(cull, device):
   rmax = _adjugate_max(v, t)[hit]
   assert (rmax < 2.0 ** 32).sum() > 20 and (rmax > 2.0 ** 34).sum() > 20
   assert (np.abs(v[t[hit]]).max((1, 2)) >= 32768).any()
+
+
+def test_prefix_mode_writes_up_to_the_first_empty_layer(device):
+  # DR_RASTER_SPARSE | DR_RASTER_PREFIX: a pixel's layers are written up to
+  # and including its first empty one, exactly as the oracle has them; the
+  # layers behind it, and background z / barycentrics, keep what the buffers
+  # held before.
+  s = scenes.icosphere_scene(seed=5)
+  clip = scenes.to_clip(s["vertices"], s["camera"])
+  L, W, H = 4, 96, 80
+  want = raster_oracle.rasterize(clip, s["triangles"], W, H, L, 0)
+  vt = torch.as_tensor(clip, device=device)
+  tt = torch.as_tensor(s["triangles"], device=device)
+  ids = torch.full((L, H, W), 12345, dtype=torch.int32, device=device)
+  z = torch.full((L, H, W), 7.0, device=device)
+  b = torch.full((L, H, W, 3), 7.0, device=device)
+  cfg = rasterize._config(
+      flags=_native.RASTER_SPARSE | _native.RASTER_PREFIX)
+  rasterize._rasterize_one(vt, tt, W, H, L, 0, cfg, out=(ids, z, b))
+  ids, z, b = ids.cpu().numpy(), z.cpu().numpy(), b.cpu().numpy()
+  frag = want[0] >= 0
+  written = np.ones((L, H, W), bool)
+  written[1:] = np.cumprod(frag[:-1], axis=0).astype(bool)
+  assert np.array_equal(ids[written], want[0][written])
+  assert (ids[~written] == 12345).all()
+  assert helpers.same_bits(z[frag], want[1][frag])
+  assert helpers.same_bits(b[frag], want[2][frag])
+  assert (z[~frag] == 7.0).all() and (b[~frag] == 7.0).all()
+  assert (~written).any() and (written & ~frag).any() and frag[1].any()
 
 
 @pytest.mark.parametrize("w,h", [(1, 1), (2048, 160), (160, 2048), (31, 1)])
