@@ -32,7 +32,12 @@
 namespace dr {
 namespace {
 
-constexpr int kBlock = 256;
+// Threads per block of every raster kernel.  Two waves: a tile block of 256
+// threads held its LDS and wave slots while its slowest wave finished, and
+// with two job groups on concurrent streams the smaller blocks interleave
+// with the other group's kernels (measured: 0.929 -> 0.905 ms per fused
+// 8-view step; 64 threads made the tile pass itself 15 % slower).
+constexpr int kBlock = 128;
 // Depth layers per pixel; the tile's L-deep key lists must also fit the
 // 64 KiB LDS budget (L x tile pixels x 8 bytes).
 constexpr int kMaxLayers = 64;
