@@ -480,16 +480,23 @@ void target_kernel(
             cv[c] = sc[i * C + c];
           }
         }
+        // The reference's masked accumulation (splat.py:147-160) as
+        // fma(tap, 0/1 class mask, acc): the bits of an add of the tap into
+        // its class and of a zero into the others (sums start at +0 and are
+        // never -0), in one instruction per class and channel.
+        const float m0 = kc == 0 ? 1.0f : 0.0f;
+        const float m1 = kc == 1 ? 1.0f : 0.0f;
+        const float m2 = kc == 2 ? 1.0f : 0.0f;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
           const float wc = w * cv[c];
-          acc[0][c] += kc == 0 ? wc : 0.0f;
-          acc[1][c] += kc == 1 ? wc : 0.0f;
-          acc[2][c] += kc == 2 ? wc : 0.0f;
+          acc[0][c] = fmaf(wc, m0, acc[0][c]);
+          acc[1][c] = fmaf(wc, m1, acc[1][c]);
+          acc[2][c] = fmaf(wc, m2, acc[2][c]);
         }
-        acc[0][C] += kc == 0 ? w : 0.0f;
-        acc[1][C] += kc == 1 ? w : 0.0f;
-        acc[2][C] += kc == 2 ? w : 0.0f;
+        acc[0][C] = fmaf(w, m0, acc[0][C]);
+        acc[1][C] = fmaf(w, m1, acc[1][C]);
+        acc[2][C] = fmaf(w, m2, acc[2][C]);
       }
     }
     __syncthreads();  // the next layer's staging overwrites sx, sy, sc

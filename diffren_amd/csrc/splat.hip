@@ -143,17 +143,25 @@ __device__ __forceinline__ void accumulate_target(const SplatFrame& f, int x,
           tap_weight(m, f.coef, f.xyz[3 * q], f.xyz[3 * q + 1], u, v, dx, dy);
       const int k = layer_class(surf[t], l);
       const float* col = f.color + q * C;
+      // The reference's masked accumulation (splat.py:147-160): each class
+      // adds the tap times its 0/1 mask, here fma(tap, mask, acc) -- one
+      // rounding of acc + tap where the mask is 1 (tap * 1 is exact), acc
+      // plus a zero where it is 0 (no bit changes: a sum that starts at +0
+      // is never -0), and a non-finite tap reaches every class as tap * 0
+      // does there.  Branch-free, so acc stays in registers.
+      const float m0 = k == 0 ? 1.0f : 0.0f;
+      const float m1 = k == 1 ? 1.0f : 0.0f;
+      const float m2 = k == 2 ? 1.0f : 0.0f;
 #pragma unroll
       for (int c = 0; c < C; ++c) {
         const float wc = w * col[c];
-        // Branch-free select of the class row keeps acc in registers.
-        acc[0][c] += k == 0 ? wc : 0.0f;
-        acc[1][c] += k == 1 ? wc : 0.0f;
-        acc[2][c] += k == 2 ? wc : 0.0f;
+        acc[0][c] = fmaf(wc, m0, acc[0][c]);
+        acc[1][c] = fmaf(wc, m1, acc[1][c]);
+        acc[2][c] = fmaf(wc, m2, acc[2][c]);
       }
-      acc[0][C] += k == 0 ? w : 0.0f;
-      acc[1][C] += k == 1 ? w : 0.0f;
-      acc[2][C] += k == 2 ? w : 0.0f;
+      acc[0][C] = fmaf(w, m0, acc[0][C]);
+      acc[1][C] = fmaf(w, m1, acc[1][C]);
+      acc[2][C] = fmaf(w, m2, acc[2][C]);
     }
   }
 }
