@@ -57,6 +57,11 @@ def parse():
   p.add_argument("--streams", type=_groups_arg, default="auto",
                  help="concurrent job groups in the fused step, or 'auto' "
                  "(the autotuner times 1, 2 and 4 groups)")
+  p.add_argument("--tile-threads",
+                 type=lambda v: v if v == "auto" else int(v), default=None,
+                 help="threads per raster tile block (128 or 256), or "
+                 "'auto' (timed on the step; the default with --streams "
+                 "auto, else 256)")
   p.add_argument("--deterministic", action="store_true",
                  help="fixed-point (order-independent) gradient sums")
   p.add_argument("--fused-transform", action="store_true",
@@ -187,7 +192,8 @@ def pmc_probe(args):
   r = batch.BatchedSplatRenderer([w["mesh"]], jobs, L, tile=(16, 16),
                                  streams=(2 if args.streams == "auto" else
                                           args.streams),
-                                 fused_transform=args.fused_transform)
+                                 fused_transform=args.fused_transform,
+                                 tile_threads=args.tile_threads)
   for t in r.targets:
     t.uniform_()
   for _ in range(3):
@@ -446,11 +452,14 @@ def main():
   r = batch.BatchedSplatRenderer([w["mesh"]], jobs, L, tile="auto",
                                  streams=args.streams,
                                  fused_transform=args.fused_transform,
-                                 deterministic=args.deterministic)
+                                 deterministic=args.deterministic,
+                                 tile_threads=args.tile_threads)
   log(f"autotuned raster tile {r.raster_config.tile_w}x"
       f"{r.raster_config.tile_h}, bin capacity {r.raster_config.bin_capacity}"
       f", timings {r.autotune_timings}; {len(r.groups)} job groups"
-      f" (step ms by group count: {r.streams_timings})")
+      f" (step ms by group count: {r.streams_timings}); tile pass in "
+      f"{r.raster_config.tile_threads}-thread blocks (step ms: "
+      f"{r.tile_threads_timings})")
   for i in range(B):
     r.targets[i].copy_(w["targets"][i])
   r.step()
@@ -592,6 +601,7 @@ def main():
                              (" in a HIP graph" if use_graph else ""),
                  "triangles": T, "vertices": V, "resolution": R, "layers": L,
                  "views": B, "channels": C, "job_groups": len(r.groups),
+                 "tile_threads": r.raster_config.tile_threads,
                  "parallelism": "single GPU"},
       "mtri_per_s": round(renders_per_s * T / 1e6, 1),
       "plain_renders_per_s": round(plain_rps, 2),

@@ -33,7 +33,7 @@ PAD_EDGE = 1
 class RasterConfig(ctypes.Structure):
   _fields_ = [("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32),
               ("bin_capacity", ctypes.c_int64), ("flags", ctypes.c_int32),
-              ("reserved", ctypes.c_int32)]
+              ("tile_threads", ctypes.c_int32)]
 
 
 class RasterJob(ctypes.Structure):

@@ -1,5 +1,5 @@
-"""Autotuner for the batched render: screen-tile shape, bin capacity and the
-number of concurrent job groups.
+"""Autotuner for the batched render: screen-tile shape, bin capacity, the
+number of concurrent job groups and the tile pass's block size.
 
 For a batch of raster jobs it times every candidate tile shape with HIP
 events on the current stream (after one warm-up launch), keeps the fastest,
@@ -8,7 +8,9 @@ needed (dr_raster_bin_stats) plus headroom, so the default over-allocation
 (3 entries per triangle) shrinks to what the workload uses.  tune_groups
 times the whole captured step for each candidate number of job groups (each
 group's kernels run on their own stream, so groups overlap) and keeps the
-fastest.  Results are cached in memory per workload signature (image sizes,
+fastest; tune_tile_threads then times that step with the tile pass in 256-
+and 128-thread blocks (128 interleaves better with the other groups' kernels
+on sparse frames, 256 is faster on dense ones).  Results are cached in memory per workload signature (image sizes,
 triangle counts, layers, culling) and optionally in a JSON file.
 """
 import ctypes
@@ -21,6 +23,7 @@ from diffren_amd import _native
 
 CANDIDATE_TILES = [(8, 8), (16, 16), (32, 8), (32, 16), (64, 8), (32, 32)]
 CANDIDATE_GROUPS = (1, 2, 4)
+CANDIDATE_TILE_THREADS = (256, 128)
 HEADROOM = 1.25
 
 _cache = {}
@@ -107,29 +110,45 @@ def tune_raster(table_host, num_jobs, num_layers, culling, device,
   return tw, th, capacity, timings
 
 
+def _tune_step(key, field, candidates, run_with, reps, warmup, cache_path):
+  """(choice, timings_ms): the candidate whose step runs fastest.
+  run_with(c) prepares the renderer for candidate c and returns a function
+  that enqueues one step (a graph replay); cached under key/field."""
+  _load(cache_path)
+  key = key + "/" + field
+  if key in _cache:
+    e = _cache[key]
+    return e[field], e["timings"]
+  timings = {}
+  best = None
+  for c in candidates:
+    fn = run_with(c)
+    for _ in range(warmup):
+      fn()
+    ms = time_launch(fn, reps)
+    timings[str(c)] = ms
+    if best is None or ms < best[0]:
+      best = (ms, c)
+  _cache[key] = {field: best[1], "timings": timings}
+  _save(cache_path)
+  return best[1], timings
+
+
 def tune_groups(key, run_with, num_jobs, reps=40, warmup=5, cache_path=None,
                 candidates=None):
   """Returns (groups, timings_ms): the number of job groups whose step runs
   fastest.  run_with(n) prepares the renderer for n groups and returns a
   function that enqueues one step (a graph replay); key identifies the
   workload (signature() plus whatever else fixes the step)."""
-  _load(cache_path)
-  key = key + "/groups"
-  if key in _cache:
-    e = _cache[key]
-    return e["groups"], e["timings"]
-  timings = {}
-  best = None
-  for n in (candidates or CANDIDATE_GROUPS):
-    if n > num_jobs:
-      continue
-    fn = run_with(n)
-    for _ in range(warmup):
-      fn()
-    ms = time_launch(fn, reps)
-    timings[str(n)] = ms
-    if best is None or ms < best[0]:
-      best = (ms, n)
-  _cache[key] = dict(groups=best[1], timings=timings)
-  _save(cache_path)
-  return best[1], timings
+  cands = [n for n in (candidates or CANDIDATE_GROUPS) if n <= num_jobs]
+  return _tune_step(key, "groups", cands, run_with, reps, warmup, cache_path)
+
+
+def tune_tile_threads(key, run_with, reps=40, warmup=5, cache_path=None,
+                      candidates=None):
+  """Returns (threads, timings_ms): the tile-pass block size
+  (dr_raster_config.tile_threads) whose step runs fastest; run_with(t) as
+  for tune_groups."""
+  return _tune_step(key, "tile_threads",
+                    list(candidates or CANDIDATE_TILE_THREADS), run_with,
+                    reps, warmup, cache_path)

@@ -53,7 +53,12 @@ class BatchedSplatRenderer:
            autotuner (diffren_amd.autotune).
   streams: number of job groups run concurrently (each on its own stream),
            or "auto": the autotuner times the captured step with 1, 2 and 4
-           groups on this batch and keeps the fastest (streams_timings).
+           groups on this batch and keeps the fastest (streams_timings),
+           then times that step with the tile pass in 256- and 128-thread
+           blocks (tile_threads_timings).
+  tile_threads: threads per tile-pass block (dr_raster_config.tile_threads):
+           128 or 256, "auto" to time both on the captured step, or None:
+           "auto" with streams="auto", else 256.
   fused_transform: when True no clip-space buffers are written: the
            rasterizer and the splat passes read the world-space vertices
            and transform them by the job's camera as they load them (same
@@ -99,7 +104,7 @@ class BatchedSplatRenderer:
                loss_scale=None, autotune_cache=None, streams=1,
                fused_transform=False, return_accum_buffers=False,
                camera_grads=False, deterministic=False, fixed_shift=40,
-               shading_function=None, channels=None):
+               shading_function=None, channels=None, tile_threads=None):
     lib = _native.lib()
     self.meshes = meshes
     self.jobs = jobs
@@ -236,6 +241,7 @@ class BatchedSplatRenderer:
     self.autotune_cache = autotune_cache
     self.autotune_timings = None
     self.streams_timings = None
+    self.tile_threads_timings = None
     auto_groups = streams == "auto"
     self._set_groups(1 if auto_groups else streams)
     # The fused passes read ids first, so background z / barycentrics are
@@ -256,6 +262,17 @@ class BatchedSplatRenderer:
     for g in self.groups:
       self._plan_raster(g)
       self._plan_splat(g)
+    if tile_threads is None:
+      tile_threads = "auto" if auto_groups else 256
+    if tile_threads == "auto":
+      self._autotune_tile_threads()
+    else:
+      if int(tile_threads) not in (128, 256):
+        raise ValueError("tile_threads must be 128, 256 or 'auto'")
+      cfg = self.raster_config
+      self.raster_config = _native.RasterConfig(
+          cfg.tile_w, cfg.tile_h, cfg.bin_capacity, cfg.flags,
+          int(tile_threads))
     # Bin-overflow watch.  Each group's rasterizer status words (entries
     # needed, overflow flag) are copied into pinned host memory at the end
     # of its raster stage, inside a captured graph too.  The next step() or
@@ -321,16 +338,41 @@ class BatchedSplatRenderer:
         self._plan_splat(g)
       return self.capture().replay
 
+    n, self.streams_timings = autotune.tune_groups(
+        self._tune_key(), run_with, len(self.jobs),
+        cache_path=self.autotune_cache)
+    self.graph = None
+    return n
+
+  def _tune_key(self):
+    """Autotuner key of this batch's step (the raster autotuner's signature
+    plus what else fixes the step)."""
+    from diffren_amd import autotune
     sizes = [(w, h, self.meshes[mi]["triangles"].shape[0])
              for mi, _, w, h in self.jobs]
-    key = autotune.signature(sizes, self.L, 1) + json.dumps(
+    return autotune.signature(sizes, self.L, 1) + json.dumps(
         [self.raster_config.tile_w, self.raster_config.tile_h, self.C,
          bool(self.fused_transform), bool(self.fixed_shift),
          self.shading_function is not None])
-    n, self.streams_timings = autotune.tune_groups(
-        key, run_with, len(self.jobs), cache_path=self.autotune_cache)
+
+  def _autotune_tile_threads(self):
+    """Tile-pass block size from diffren_amd.autotune.tune_tile_threads: the
+    captured step is timed with each candidate on this batch and group
+    count."""
+    from diffren_amd import autotune
+    cfg = self.raster_config
+
+    def run_with(t):
+      self.raster_config = _native.RasterConfig(
+          cfg.tile_w, cfg.tile_h, cfg.bin_capacity, cfg.flags, t)
+      return self.capture().replay
+
+    t, self.tile_threads_timings = autotune.tune_tile_threads(
+        self._tune_key() + f"/{len(self.groups)}", run_with,
+        cache_path=self.autotune_cache)
+    self.raster_config = _native.RasterConfig(
+        cfg.tile_w, cfg.tile_h, cfg.bin_capacity, cfg.flags, t)
     self.graph = None
-    return n
 
   def _split(self, n):
     """Splits jobs into n groups of balanced pixel counts (greedy)."""
@@ -466,7 +508,7 @@ class BatchedSplatRenderer:
     cfg = self.raster_config
     if half:
       cfg = _native.RasterConfig(cfg.tile_w, cfg.tile_h, cfg.bin_capacity,
-                                 cfg.flags | half, 0)
+                                 cfg.flags | half, cfg.tile_threads)
     _native.check(
         _native.lib().dr_rasterize_triangles_batched(
             _native.ptr(g.raster_table), len(g.indices), g.total_triangles,
@@ -647,7 +689,7 @@ class BatchedSplatRenderer:
     cap = max(int(2 * need) + 4096, int(self.raster_config.bin_capacity))
     self.raster_config = _native.RasterConfig(
         self.raster_config.tile_w, self.raster_config.tile_h, cap,
-        self.raster_config.flags, 0)
+        self.raster_config.flags, self.raster_config.tile_threads)
     for g in self.groups:
       self._plan_raster(g)
     self.regrows += 1

@@ -101,6 +101,7 @@ def run(args):
       "raster_tile": [r.raster_config.tile_w, r.raster_config.tile_h],
       "bin_capacity": int(r.raster_config.bin_capacity),
       "streams": len(r.groups),
+      "tile_threads": r.raster_config.tile_threads,
       "streams_timings": r.streams_timings,
   }
 

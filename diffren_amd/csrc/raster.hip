@@ -32,11 +32,10 @@
 namespace dr {
 namespace {
 
-// Threads per block of every raster kernel.  Two waves: a tile block of 256
-// threads held its LDS and wave slots while its slowest wave finished, and
-// with two job groups on concurrent streams the smaller blocks interleave
-// with the other group's kernels (measured: 0.929 -> 0.905 ms per fused
-// 8-view step; 64 threads made the tile pass itself 15 % slower).
+// Threads per block of the per-triangle and scan kernels: two waves, so
+// with two job groups on concurrent streams they interleave with the other
+// group's kernels.  The tile pass takes 256 or 128 (dr_raster_config
+// tile_threads).
 constexpr int kBlock = 128;
 // Depth layers per pixel; the tile's L-deep key lists must also fit the
 // 64 KiB LDS budget (L x tile pixels x 8 bytes).
@@ -1008,8 +1007,8 @@ struct TriRect {
 // set up here and its pixels walked: by its own lane when its clipped rect
 // is small, else queued and walked by a whole wave, one pixel per lane.
 // Five waves per SIMD (at most 96 VGPRs).
-template <int TW, int TH>
-__global__ __launch_bounds__(kBlock)
+template <int TW, int TH, int kThreads>
+__global__ __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
     Jobs J, int L, int cull, const uint32_t* __restrict__ counts,
     const uint32_t* __restrict__ offsets,
@@ -1018,7 +1017,7 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
     uint64_t capacity, bool sparse, bool prefix) {
   constexpr int TP = TW * TH;
   extern __shared__ uint64_t keys[];  // [L][TP]
-  __shared__ TriRect queue[kBlock];
+  __shared__ TriRect queue[kThreads];
   __shared__ int queue_n;
   __shared__ uint32_t ties[(TP + 31) / 32];
   __shared__ uint64_t cand[kMaxCandidates];
@@ -1054,7 +1053,7 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
   const int64_t P = int64_t(W) * H;
   if (cnt == 0) {
     // Empty tile (most of a frame): background everywhere, no LDS.
-    for (int p = threadIdx.x; p < TP; p += kBlock) {
+    for (int p = threadIdx.x; p < TP; p += kThreads) {
       const int ix = x0 + p % TW, iy = y0 + p / TW;
       if (ix >= x1 || iy >= y1) continue;
       const int64_t pix = int64_t(iy) * W + ix;
@@ -1074,10 +1073,10 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
     }
     return;
   }
-  for (int i = threadIdx.x; i < L * TP; i += kBlock) keys[i] = kEmpty;
-  for (int i = threadIdx.x; i < (TP + 31) / 32; i += kBlock) ties[i] = 0u;
+  for (int i = threadIdx.x; i < L * TP; i += kThreads) keys[i] = kEmpty;
+  for (int i = threadIdx.x; i < (TP + 31) / 32; i += kThreads) ties[i] = 0u;
 
-  for (int64_t base = 0; base < items; base += kBlock) {
+  for (int64_t base = 0; base < items; base += kThreads) {
     if (threadIdx.x == 0) queue_n = 0;
     __syncthreads();
     const int64_t i = base + threadIdx.x;
@@ -1155,7 +1154,7 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
     __syncthreads();
     // Each wave walks whole queued triangles, one pixel per lane.
     const int nq = queue_n;
-    for (int q = threadIdx.x / 64; q < nq; q += kBlock / 64) {
+    for (int q = threadIdx.x / 64; q < nq; q += kThreads / 64) {
       const TriRect rq = queue[q];
       float4 v[3];
       load_triangle(job, rq.tri, v);
@@ -1192,7 +1191,7 @@ __attribute__((amdgpu_waves_per_eu(5))) void raster_tile_kernel(
   }
 
   // Resolve: recompute the winning fragments exactly and write the outputs.
-  for (int p = threadIdx.x; p < TP; p += kBlock) {
+  for (int p = threadIdx.x; p < TP; p += kThreads) {
     const int ix = x0 + p % TW, iy = y0 + p / TW;
     if (ix >= x1 || iy >= y1) continue;
     const int64_t pix = int64_t(iy) * W + ix;
@@ -1291,7 +1290,7 @@ uint64_t capacity_of(const dr_raster_config& c, int64_t total_tris,
 template <int TW, int TH>
 int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
                int cull, const Workspace& w, uint64_t capacity, int flags,
-               hipStream_t s) {
+               int tile_threads, hipStream_t s) {
   // The two halves can run as separate calls on the same workspace (each
   // call re-reads the job table): binning, then the tile pass.
   const bool run_bins = (flags & DR_RASTER_TILES_ONLY) == 0;
@@ -1325,19 +1324,23 @@ int launch_all(const Jobs& J, int64_t total_tris, int64_t total_tiles, int L,
   }
   if (!run_tiles) return kOk;
   const size_t lds = sizeof(uint64_t) * L * TW * TH;
-  if (lds > 32768) {
-    // Deep key lists: dynamic LDS plus the kernel's static arrays may pass
-    // the default 64 KiB launch limit.
-    DR_CHECK_HIP(hipFuncSetAttribute(
-        reinterpret_cast<const void*>(&raster_tile_kernel<TW, TH>),
-        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-  }
-  raster_tile_kernel<TW, TH><<<unsigned(total_tiles), kBlock, lds, s>>>(
-      J, L, cull, w.counts, w.offsets, w.bin_key, w.bin_pix, w.head,
-      reinterpret_cast<const uint32_t*>(w.frag_z), capacity,
-      (flags & DR_RASTER_SPARSE) != 0, (flags & DR_RASTER_PREFIX) != 0);
-  DR_CHECK_LAUNCH("raster tiles");
-  return kOk;
+  auto launch = [&](auto kernel, int threads) -> int {
+    if (lds > 32768) {
+      // Deep key lists: dynamic LDS plus the kernel's static arrays may pass
+      // the default 64 KiB launch limit.
+      DR_CHECK_HIP(hipFuncSetAttribute(
+          reinterpret_cast<const void*>(kernel),
+          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    }
+    kernel<<<unsigned(total_tiles), threads, lds, s>>>(
+        J, L, cull, w.counts, w.offsets, w.bin_key, w.bin_pix, w.head,
+        reinterpret_cast<const uint32_t*>(w.frag_z), capacity,
+        (flags & DR_RASTER_SPARSE) != 0, (flags & DR_RASTER_PREFIX) != 0);
+    DR_CHECK_LAUNCH("raster tiles");
+    return kOk;
+  };
+  return tile_threads == 128 ? launch(&raster_tile_kernel<TW, TH, 128>, 128)
+                             : launch(&raster_tile_kernel<TW, TH, 256>, 256);
 }
 
 // `table` is a device table of `count` jobs, or null with `single` the one
@@ -1353,6 +1356,9 @@ int dispatch(const dr_raster_job* table, const dr_raster_job* single,
                "DR_RASTER_BINS_ONLY and DR_RASTER_TILES_ONLY exclude each "
                "other");
   DR_CHECK_ARG(valid_tile(c.tile_w, c.tile_h), "unsupported tile size");
+  DR_CHECK_ARG(c.tile_threads == 0 || c.tile_threads == 128 ||
+                   c.tile_threads == 256,
+               "tile_threads must be 0, 128 or 256");
   DR_CHECK_ARG(total_tiles > 0 && total_tiles < (int64_t(1) << 31),
                "tile count out of range");
   DR_CHECK_ARG(total_tris >= 0 && total_tris < (int64_t(1) << 40),
@@ -1378,7 +1384,7 @@ int dispatch(const dr_raster_job* table, const dr_raster_job* single,
 #define DR_TILE_CASE(A, B)                                              \
   if (c.tile_w == A && c.tile_h == B)                                   \
     return launch_all<A, B>(J, total_tris, total_tiles, L, cull, w,     \
-                            capacity, c.flags, s);
+                            capacity, c.flags, c.tile_threads, s);
   DR_TILE_CASE(8, 8)
   DR_TILE_CASE(16, 16)
   DR_TILE_CASE(32, 8)
@@ -1441,6 +1447,9 @@ int dr_raster_jobs_prepare(dr_raster_job* jobs, int32_t n,
   const dr_raster_config c = dr::resolve_config(config);
   DR_CHECK_ARG(jobs != nullptr && n > 0, "need at least one job");
   DR_CHECK_ARG(dr::valid_tile(c.tile_w, c.tile_h), "unsupported tile size");
+  DR_CHECK_ARG(c.tile_threads == 0 || c.tile_threads == 128 ||
+                   c.tile_threads == 256,
+               "tile_threads must be 0, 128 or 256");
   int64_t tri = 0, tiles = 0;
   for (int i = 0; i < n; ++i) {
     dr_raster_job& j = jobs[i];

@@ -109,6 +109,7 @@ def run(args):
           "achieved_gbs": round(achieved, 1),
           "frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 4)},
       "streams": len(r.groups),
+      "tile_threads": r.raster_config.tile_threads,
   }
 
 

@@ -87,7 +87,12 @@ typedef struct dr_raster_config {
   int32_t tile_h;
   int64_t bin_capacity;
   int32_t flags;     /* DR_RASTER_* bits */
-  int32_t reserved;  /* 0 */
+  int32_t tile_threads; /* threads per tile-pass block: 0 (= 256) or 128.
+                           256 suits dense frames (hundreds of bin entries
+                           per tile); 128 lets the tile pass interleave with
+                           kernels running on another stream (the batched
+                           renderer's job groups) on sparse ones.  Results
+                           are identical. */
 } dr_raster_config;
 
 /* z and barycentrics are written only where a triangle was stored

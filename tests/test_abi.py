@@ -51,6 +51,14 @@ def test_host_only_entry_points():
   assert rc != 0 and b"tile" in lib.dr_last_error()
   ok = lib.dr_raster_jobs_prepare(job, 1, ctypes.byref(cfg), None, None)
   assert ok == 0
+  # The tile pass's block size: 0 (256), 128 or 256.
+  for t, good in ((0, True), (128, True), (256, True), (64, False),
+                  (512, False)):
+    c = _native.RasterConfig(32, 8, 0, 0, t)
+    rc = lib.dr_raster_jobs_prepare(job, 1, ctypes.byref(c), None, None)
+    assert (rc == 0) == good, t
+    if not good:
+      assert b"tile_threads" in lib.dr_last_error()
   assert lib.dr_splat_bwd_workspace_size(4, 5, 3) == 4 * 5 * 3 * 4 * 4
 
 

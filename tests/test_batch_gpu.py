@@ -124,9 +124,9 @@ def test_graph_replays_match_eager(streams, device):
 
 def test_autotuned_group_count(device, tmp_path):
   """streams="auto": the step is timed with 1, 2 and 4 job groups (as many
-  as there are jobs), the
-  fastest count is kept (and cached), and the step's results equal a
-  renderer built with that count."""
+  as there are jobs), the fastest count is kept (and cached), then the tile
+  pass's block size likewise; the step's results equal a renderer built with
+  that count and the other block size."""
   meshes = _meshes(device)
   jobs = _jobs(meshes)
   cache = str(tmp_path / "tune.json")
@@ -136,7 +136,12 @@ def test_autotuned_group_count(device, tmp_path):
   assert set(r.streams_timings) == {"1", "2"}
   best = min(r.streams_timings, key=r.streams_timings.get)
   assert len(r.groups) == int(best)
-  f = batch.BatchedSplatRenderer(meshes, jobs, 3, streams=int(best))
+  # Then the tile pass's block size, timed on the chosen step.
+  assert set(r.tile_threads_timings) == {"128", "256"}
+  threads = int(min(r.tile_threads_timings, key=r.tile_threads_timings.get))
+  assert r.raster_config.tile_threads == threads
+  f = batch.BatchedSplatRenderer(meshes, jobs, 3, streams=int(best),
+                                 tile_threads=384 - threads)
   for ta, tb in zip(r.targets, f.targets):
     ta.uniform_()
     tb.copy_(ta)
@@ -154,6 +159,10 @@ def test_autotuned_group_count(device, tmp_path):
                                      autotune_cache=cache)
   assert len(again.groups) == len(r.groups)
   assert again.streams_timings == r.streams_timings
+  assert again.tile_threads_timings == r.tile_threads_timings
+  assert again.raster_config.tile_threads == threads
+  with pytest.raises(ValueError):
+    batch.BatchedSplatRenderer(meshes, jobs, 3, tile_threads=64)
 
 
 @pytest.mark.parametrize("streams", [1, 4])

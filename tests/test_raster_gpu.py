@@ -65,6 +65,28 @@ def test_tile_shapes_bit_exact(tile, device):
                 raster_oracle.rasterize(v, t, w, h, l, cull))
 
 
+@pytest.mark.parametrize("threads", [128, 256])
+@pytest.mark.parametrize("tile", [(8, 8), (32, 8), (32, 32)])
+def test_tile_block_sizes_bit_exact(tile, threads, device):
+  # dr_raster_config.tile_threads: the tile pass in 128- or 256-thread
+  # blocks, on scenes with large triangles (wave-walked queues), dense
+  # overdraw (entry loops of several passes) and exact depth ties.
+  rng = np.random.default_rng(threads + tile[0])
+  scenes_ = [golden_scenes.adversarial_scene(300 + threads, 80, 400),
+             _stack(60, rng, coplanar=True), _stack(90, rng, coplanar=False)]
+  for k, (v, t) in enumerate(scenes_):
+    w, h, l = [(96, 80, 3), (48, 40, 8), (64, 56, 4)][k]
+    cfg = rasterize._config(tile, 0, 0, l)
+    cfg.tile_threads = threads
+    vt = torch.as_tensor(np.ascontiguousarray(v, np.float32), device=device)
+    tt = torch.as_tensor(np.ascontiguousarray(t, np.int32), device=device)
+    print(f"tile {tile} threads {threads} scene {k}", flush=True)
+    ids, z, b = rasterize._rasterize_one(vt, tt, w, h, l, 0, cfg)
+    torch.cuda.synchronize()
+    assert_same((ids.cpu().numpy(), z.cpu().numpy(), b.cpu().numpy()),
+                raster_oracle.rasterize(v, t, w, h, l, 0))
+
+
 @pytest.mark.parametrize("seed", range(10))
 def test_random_scenes_bit_exact(seed, device):
   rng = np.random.default_rng(seed)
