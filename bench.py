@@ -516,8 +516,9 @@ def main():
                              P * 4 * C) + V * 4 * (3 + C),
   }
   kernel_names = {
-      "raster_tiles": "raster_tile_kernel<%d, %d>" % (
-          r.raster_config.tile_w, r.raster_config.tile_h),
+      "raster_tiles": "raster_tile_kernel<%d, %d, %d>" % (
+          r.raster_config.tile_w, r.raster_config.tile_h,
+          r.raster_config.tile_threads or 256),
       "splat_target": "target_kernel<4, false>",
       "splat_source": "source_kernel<4, false>",
   }
