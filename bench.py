@@ -224,14 +224,14 @@ def plain_step(w, args, fused=False):
   return vv.grad, cc.grad
 
 
-def cpu_baseline(w, args):
-  """The workload's fwd+bwd on the host's cores: the reference CPU
-  rasterizer (its own source compiled here, single-threaded per image as
-  rasterize_triangles_impl.cc:37 is; the views run concurrently, one per
-  thread) and the float32 C restatement of interpolate -> splat -> over ->
-  L2 -> backward (oracle/splat_cpu.c) on all threads, one view after
-  another.  renders/s = views / (parallel raster time + views x splat
-  time); the splat time is the median of three runs of view 0."""
+def cpu_baseline(w, args, reps=3):
+  """The workload's fwd+bwd on the host's cores, timed `reps` times over
+  all of its views: the reference CPU rasterizer (its own source compiled
+  here, single-threaded per image as rasterize_triangles_impl.cc:37 is; the
+  views run concurrently, one per thread), then for each view the float32 C
+  restatement of interpolate -> splat -> over -> L2 -> backward
+  (oracle/splat_cpu.c) on all threads.  renders/s = views / the median
+  step time."""
   import numpy as np
   import torch
   from oracle import raster_oracle
@@ -242,40 +242,46 @@ def cpu_baseline(w, args):
       raster_oracle.rasterize
   R, L, B = args.resolution, args.layers, len(w["cams"])
   threads = int(torch.get_num_threads())
-  tgt = w["targets"][0].cpu().numpy()
   t0 = time.perf_counter()
   clip = so.transform(w["cams"][0], w["v"], torch.float32)
-  ids, z, b = ras(clip.numpy(), w["t"], R, R, L, 1)
+  ras(clip.numpy(), w["t"], R, R, L, 1)
   raster_ms = (time.perf_counter() - t0) * 1e3
-  times = []
-  for _ in range(3):
+  targets = [t.cpu().numpy() for t in w["targets"]]
+  steps, raster_par, splat_ms = [], None, []
+  for _ in range(reps):
     t1 = time.perf_counter()
-    splat_cpu.render(w["v"], w["colors"], w["cams"][0], w["t"], ids, b, tgt,
-                     1.0 / (R * R * 4), threads=threads)
-    times.append(time.perf_counter() - t1)
-  splat_ms = float(np.median(times)) * 1e3
-  raster_par = cpu_raster_parallel(w, args, ras)
-  total_ms = raster_par["ms"] + B * splat_ms
-  return dict(value=B / (total_ms * 1e-3), unit="renders/s", cores=threads,
+    raster_par, frames = cpu_raster_parallel(w, args, ras)
+    t2 = time.perf_counter()
+    for k, (ids, _, b) in enumerate(frames):
+      splat_cpu.render(w["v"], w["colors"], w["cams"][k], w["t"], ids, b,
+                       targets[k], 1.0 / (R * R * 4),
+                       threads=threads)
+    t3 = time.perf_counter()
+    steps.append((t3 - t1) * 1e3)
+    splat_ms.append((t3 - t2) * 1e3 / B)
+  step_ms = float(np.median(steps))
+  splat_view_ms = float(np.median(splat_ms))
+  return dict(value=B / (step_ms * 1e-3), unit="renders/s", cores=threads,
               kind="port",
-              sample=(f"the {B} views of the workload, fwd+bwd: the "
-                      f"{kind} CPU rasterizer one view per thread on "
+              sample=(f"{reps} timed runs of the workload's {B} views, "
+                      f"fwd+bwd (median {step_ms:.0f} ms): the {kind} CPU "
+                      f"rasterizer one view per thread on "
                       f"{raster_par['threads']} threads "
                       f"({raster_par['ms']:.0f} ms for {B}; "
-                      f"{raster_ms:.0f} ms for one view alone) + the C "
-                      f"float32 interpolate/splat/over/L2 fwd+bwd "
-                      f"restatement on {threads} threads ({splat_ms:.0f} ms "
-                      f"per view)"),
-              raster_ms=raster_ms, splat_ms=splat_ms,
+                      f"{raster_ms:.0f} ms for one view alone), then each "
+                      f"view's C float32 interpolate/splat/over/L2 fwd+bwd "
+                      f"restatement on {threads} threads "
+                      f"({splat_view_ms:.0f} ms per view)"),
+              raster_ms=raster_ms, splat_ms=splat_view_ms, step_ms=step_ms,
               raster_kind=kind, raster_parallel=raster_par)
 
 
 def cpu_raster_parallel(w, args, ras):
   """The CPU rasterizer on all views at once, one single-threaded image per
   host thread (the reference kernel is single-threaded per image,
-  rasterize_triangles_impl.cc:37): views/s and the thread count.  ctypes
-  releases the interpreter lock during each call, so the images run
-  concurrently."""
+  rasterize_triangles_impl.cc:37): views/s and the thread count, and the
+  views' (ids, z, barycentrics).  ctypes releases the interpreter lock
+  during each call, so the images run concurrently."""
   from concurrent.futures import ThreadPoolExecutor
   import torch
   from oracle import splat_oracle as so
@@ -285,9 +291,10 @@ def cpu_raster_parallel(w, args, ras):
   threads = max(1, min(n, int(torch.get_num_threads())))
   with ThreadPoolExecutor(threads) as pool:
     t0 = time.perf_counter()
-    list(pool.map(lambda c: ras(c, w["t"], R, R, L, 1), clips))
+    frames = list(pool.map(lambda c: ras(c, w["t"], R, R, L, 1), clips))
     dt = time.perf_counter() - t0
-  return dict(views=n, threads=threads, ms=dt * 1e3, views_per_s=n / dt)
+  return dict(views=n, threads=threads, ms=dt * 1e3, views_per_s=n / dt), \
+      frames
 
 
 def finite_differences(r, vertices, target, loss_scale, count=64, eps=1e-3,
